@@ -1,0 +1,83 @@
+/*
+ * cc_oracle.h -- CPU restatement of one time step of the compressible channel
+ * DNS chanCompr (reference: v1.10/codes/main_v1.9.f90, the halo-corrected copy
+ * of v1.9/codes/main_v1.9.f90, plus v1.9/codes/comdata.f90).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path (compchan3dmpi_amd, the HIP kernels) never links or calls it.
+ *
+ * Layout and evaluation order follow the reference exactly: Fortran
+ * column-major arrays, solV(0:n1+1,0:n2+1,0:n3+1,5,2), var(0:n1+1,...,7),
+ * point fluxes fD/gD/hD(n1,n2,n3,5), loops k-j-i with i innermost, and every
+ * arithmetic expression evaluated left to right as the Fortran source groups
+ * it.  Build with -ffp-contract=off so no multiply-add is fused.
+ */
+#ifndef CC_ORACLE_H
+#define CC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc_t;
+
+/* Mesh, one-sided coefficients and constants (comdata.f90:70-134,
+ * main_v1.9.f90:52-76).  Pex/Pey/Pez are the cell-Peclet thresholds. */
+orc_t *orc_create(int n1, int n2, int n3, double Re, double Mac, double dt,
+                  int istat, double Pex, double Pey, double Pez);
+void orc_destroy(orc_t *o);
+
+/* Initial state from a primitive field var(n1,n2,n3,7) in Fortran order
+ * (rho,u,v,w,E,T,p), as the reader leaves it (main_v1.9.f90:227-251), then
+ * conserved variables, mu, kt (:284-294) and the first halo fill (:297-313). */
+int orc_load(orc_t *o, const double *var, int nstep, double time);
+
+/* x3 as read back from the solution file (F10.6); only the bulk diagnostics
+ * read it (main_v1.9.f90:841, 850).  Defaults to the analytic mesh. */
+void orc_set_x3_file(orc_t *o, const double *x3);
+
+/* Forcing state (tauxzL, tauxzU, stat1Spc(:,1)); zero until the first
+ * statistics step, exactly as the reference's static initialisation. */
+void orc_set_forcing(orc_t *o, double tauL, double tauU, const double *rhobar);
+
+/* One RK stage (iter = 1 or 2): calcFlux, calcSolV, share2P(solV), calcPrim,
+ * share2P(var), calcBC (main_v1.10.f90:37-44). */
+void orc_stage(orc_t *o, int iter);
+
+/* nsteps full time steps including the blow-up check and the istat-cadence
+ * wall-shear and plane statistics.  Returns 0, or 1 when rho<0 or T<0 was
+ * found (the step at which the reference would stop). */
+int orc_step(orc_t *o, int nsteps);
+
+/* Accessors.  var/solv are returned over 1..n in Fortran order
+ * var(n1,n2,n3,7), U(n1,n2,n3,5) (current solution, level 1). */
+void orc_get_var(const orc_t *o, double *var);
+void orc_get_solv(const orc_t *o, int level, double *U);
+void orc_get_mu(const orc_t *o, double *mu);
+int orc_get_nstep(const orc_t *o);
+double orc_get_time(const orc_t *o);
+/* tau[2] = tauxzL, tauxzU; stat1/stat2 (n3,7), corr (n3,9), Fortran order. */
+void orc_get_stats(const orc_t *o, double *tau, double *stat1, double *stat2,
+                   double *corr);
+/* coef[12] = f1d2c0..2, f2d1c0..2, b1d2c0..2, b2d1c0..2; mesh[n3*4] =
+ * x3, jac, invJac, Xi; scal[8] = dx, dy, dXi, dz1, dzN, ggM, invRe, Pi. */
+void orc_get_mesh(const orc_t *o, double *coef, double *mesh, double *scal);
+/* Residual R(n1,n2,n3,5) of the last stage (updated nodes only, 0 elsewhere). */
+void orc_get_residual(const orc_t *o, double *R);
+/* Blow-up location of the last orc_step that returned 1: i,j,k (1-based). */
+void orc_get_blown(const orc_t *o, int *ijk);
+
+/* Bulk diagnostics of main_v1.9.f90:799-906 on the current var:
+ * out[0] = KE, out[1..5] rho_b, out[6..10] U_b, out[11..15] massflow for the
+ * planes [5,10,70,100,125]; a plane outside 2 < p < n1-1 gives 0. */
+void orc_diag(const orc_t *o, double *out);
+
+/* Run the reference's plane-statistics / wall-shear routines on demand. */
+void orc_calc_tauwall(orc_t *o);
+void orc_avg_spatial(orc_t *o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

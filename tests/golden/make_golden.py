@@ -1,0 +1,65 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference
+solver (v1.10/codes/main_v1.9.f90 + v1.9/codes/comdata.f90, compiled from
+/root/reference by oracle/build_ref.sh) on seeded inputs.
+
+Run in a container where /root/reference exists:
+
+    python tests/golden/make_golden.py
+
+Fixtures (all inputs are regenerated from the seed by the tests):
+  tiny_12x8x16_s1.npy     var(7,16,8,12) after 1 step,  dt=1e-3, istat=100
+  tiny_12x8x16_s10.npy    var after 10 steps, dt=1e-3, istat=5 (forcing on)
+  tiny_12x8x16_s10_tau.npy   tauWall.dat rows of that run
+  c1_64x32x48_s100.npz    config 1 after 100 steps (dt=0.0015, istat=10):
+                          sha256 of the full var bytes, per-field sums, L2,
+                          Linf, three full planes, tauWall/moment1st rows
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from compchan3dmpi_amd import formats  # noqa: E402
+import ref_run  # noqa: E402
+
+
+def ic(n1, n2, n3):
+    return formats.round_e22_15(formats.laminar_ic(n1, n2, n3))
+
+
+def main():
+    # tiny grid, 1 step, forcing off
+    v = ic(12, 8, 16)
+    r = ref_run.run(v, formats.Params(dt=0.001, istat=100), 1, 1)
+    np.save(os.path.join(HERE, "tiny_12x8x16_s1.npy"), r["var"])
+    # tiny grid, 10 steps, statistics every 5 steps (forcing switches on)
+    r = ref_run.run(v, formats.Params(dt=0.001, istat=5), 10, 10)
+    np.save(os.path.join(HERE, "tiny_12x8x16_s10.npy"), r["var"])
+    np.save(os.path.join(HERE, "tiny_12x8x16_s10_tau.npy"), r["tauWall"])
+    # config 1
+    v = ic(64, 32, 48)
+    r = ref_run.run(v, formats.Params(dt=0.0015, istat=10), 100, 100)
+    var = np.ascontiguousarray(r["var"])
+    np.savez(
+        os.path.join(HERE, "c1_64x32x48_s100.npz"),
+        sha256=np.frombuffer(hashlib.sha256(var.tobytes()).digest(), dtype=np.uint8),
+        sums=var.reshape(7, -1).sum(axis=1),
+        l2=np.sqrt((var.reshape(7, -1) ** 2).sum(axis=1)),
+        linf=np.abs(var.reshape(7, -1)).max(axis=1),
+        plane_k2=var[:, 1], plane_k24=var[:, 23], plane_j16=var[:, :, 15],
+        tauWall=r["tauWall"], moment1st=r["moment1st"],
+        KE=r["KE"], rhobulk=r["rhobulk"], Ubulk=r["Ubulk"], massflow=r["massflow"],
+    )
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

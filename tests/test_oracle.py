@@ -1,0 +1,122 @@
+"""The CPU restatement (oracle) against the reference's golden vectors, the
+reference's own known-answer test, and closed forms."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from compchan3dmpi_amd import formats
+import oracle as orc
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def ic(n1, n2, n3):
+    return formats.round_e22_15(formats.laminar_ic(n1, n2, n3))
+
+
+def test_tiny_one_step_bit_exact():
+    o = orc.Oracle(12, 8, 16, dt=0.001, istat=100)
+    o.load(ic(12, 8, 16))
+    assert o.step(1) == 0
+    ref = np.load(os.path.join(GOLD, "tiny_12x8x16_s1.npy"))
+    assert np.array_equal(o.var(), ref)
+
+
+def test_tiny_ten_steps_forcing_bit_exact():
+    o = orc.Oracle(12, 8, 16, dt=0.001, istat=5)
+    o.load(ic(12, 8, 16))
+    assert o.step(10) == 0
+    ref = np.load(os.path.join(GOLD, "tiny_12x8x16_s10.npy"))
+    assert np.array_equal(o.var(), ref)
+    tau_rows = np.load(os.path.join(GOLD, "tiny_12x8x16_s10_tau.npy"))
+    tau = o.stats()[0]
+    # tauWall.dat is written F15.7 (main_v1.9.f90:962-965)
+    assert np.allclose(tau, tau_rows[-1, 1:], atol=6e-8, rtol=0)
+
+
+def test_c1_hundred_steps_bit_exact():
+    g = np.load(os.path.join(GOLD, "c1_64x32x48_s100.npz"))
+    o = orc.Oracle(64, 32, 48, dt=0.0015, istat=10)
+    o.load(ic(64, 32, 48))
+    assert o.step(100) == 0
+    v = np.ascontiguousarray(o.var())
+    assert hashlib.sha256(v.tobytes()).digest() == g["sha256"].tobytes()
+    assert np.array_equal(v[:, 1], g["plane_k2"])
+    assert np.array_equal(v[:, 23], g["plane_k24"])
+    tau, s1, _, _ = o.stats()
+    assert np.allclose(tau, g["tauWall"][-1, 1:], atol=6e-8, rtol=0)
+    m1 = g["moment1st"][-1, 1:].reshape(48, 7).T
+    assert np.allclose(s1, m1, atol=6e-8, rtol=0)
+    d = o.diag()
+    assert np.allclose(d[0], g["KE"][-1, 1], atol=6e-8)
+
+
+def test_one_sided_coefficients_known_answer():
+    """mainV1.8_serial.f90:76-105 applies the one-sided coefficients to
+    p(x)=x^3-5x^2+3x-10 on the 160-node cosine mesh; its log
+    (v1.8_validation/codes/nohupDebug.out:23-26) prints these values."""
+    logged = [2.99999984762053, -5.00000015238402, -9.99804821610451, 1.99804823100567]
+    o = orc.Oracle(16, 16, 160)
+    coef, mesh, _ = o.mesh()
+    x3 = mesh[0]
+    pl = x3 ** 3 - 5 * x3 ** 2 + 3 * x3 - 10
+    f1 = coef[0] * pl[0] + coef[1] * pl[1] + coef[2] * pl[2]
+    b1 = coef[6] * pl[-1] + coef[7] * pl[-2] + coef[8] * pl[-3]
+    f2 = coef[3] * pl[0] + coef[4] * pl[1] + coef[5] * pl[2]
+    b2 = coef[9] * pl[-1] + coef[10] * pl[-2] + coef[11] * pl[-3]
+    assert np.allclose([f1, b1, f2, b2], logged, rtol=1e-7, atol=0)
+    # and the closed forms: exact on quadratics
+    q = x3 ** 2 - 3 * x3 + 1
+    assert abs(coef[0] * q[0] + coef[1] * q[1] + coef[2] * q[2] - (-3.0)) < 1e-9
+    assert abs(coef[3] * q[0] + coef[4] * q[1] + coef[5] * q[2] - 2.0) < 1e-6
+
+
+def test_quiescent_isothermal_state_is_steady():
+    """u=v=w=0, rho=T=1 uniform: every flux vanishes away from walls, so one
+    step leaves the state unchanged (mathematics, not a re-typed formula)."""
+    n1, n2, n3 = 10, 8, 12
+    var = np.zeros((7, n3, n2, n1))
+    var[0] = 1.0
+    var[5] = 1.0
+    var[4] = 1.0
+    var[6] = 1.0 / (1.4 * 1.5 ** 2)
+    o = orc.Oracle(n1, n2, n3, dt=1e-3, istat=100)
+    o.load(var)
+    o.step(3)
+    v = o.var()
+    assert np.allclose(v[1:4], 0.0, atol=1e-14)
+    assert np.allclose(v[0], 1.0, atol=1e-12)
+
+
+def test_periodic_shift_invariance():
+    """Shifting the periodic x/y content by m nodes (period n-2) commutes with
+    the step: halo fills are a true periodic wrap."""
+    n1, n2, n3 = 12, 10, 12
+    v = ic(n1, n2, n3)
+
+    def shift(a, m, axis):
+        core = np.take(a, np.arange(1, a.shape[axis] - 1), axis=axis)
+        core = np.roll(core, m, axis=axis)
+        n = a.shape[axis]
+        idx_lo = np.take(core, [n - 3], axis=axis)
+        idx_hi = np.take(core, [0], axis=axis)
+        return np.concatenate([idx_lo, core, idx_hi], axis=axis)
+
+    vs = shift(shift(v, 3, 3), 2, 2)
+    a = orc.Oracle(n1, n2, n3, dt=1e-3, istat=100)
+    a.load(v)
+    a.step(2)
+    b = orc.Oracle(n1, n2, n3, dt=1e-3, istat=100)
+    b.load(vs)
+    b.step(2)
+    ra = shift(shift(a.var(), 3, 3), 2, 2)
+    rb = b.var()
+    # interior nodes only: corners are never refreshed by the reference
+    sl = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    assert np.allclose(ra[sl], rb[sl], rtol=1e-13, atol=1e-13)
+
+
+def test_cfl_dt_configs():
+    assert formats.cfl_dt(formats.laminar_ic(64, 32, 48), 1.5) == 0.0015
