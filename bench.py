@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Throughput of the channel-DNS time step on one MI355X.
+
+Workload: the reference's physics (Ma 1.5, Re 3000, gamma 1.4, Pr 0.71,
+mu = T^0.7, isothermal no-slip walls, periodic x/z) on the
+256x192x128 grid = reference nP = [256, 128, 192] (6,291,456 nodes), from a
+seeded perturbed laminar start, dt from CFL 0.5.  A "step" is one full time
+step: two RK stages plus the istat-cadence wall-shear and plane statistics
+that feed the forcing, all on the device.
+
+Prints one JSON line.  value = cell-updates/s per RK stage in FP64
+(= n1*n2*n3 * 2 * steps / t); the mixed-precision figure is reported in
+config.mixed.  roofline: achieved algorithmic HBM bandwidth of the stage
+kernel (100 B per node per stage: stage 1 reads U^n and writes U*, stage 2
+reads U*, U^n and writes U^{n+1}; 5 FP64 words each) over its average launch
+time measured with HIP events on the library's stream.  cpu_baseline: the
+CPU restatement (oracle/, one core) on a bounded sample of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from compchan3dmpi_amd import formats  # noqa: E402
+from compchan3dmpi_amd.channel import Channel  # noqa: E402
+
+METRIC = "cell-updates/s per RK stage (FP64 vs mixed) + % HBM roofline, 1 MI355X"
+NP = (256, 128, 192)
+DT = 9e-5           # CFL 0.5 on the seeded start, five decimals (formats.cfl_dt)
+ISTAT = 10
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_PER_NODE_STAGE = 100.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(var, precision, steps, warmup, profile):
+    n1, n2, n3 = NP
+    ch = Channel(n1, n2, n3, dt=DT, istat=ISTAT, precision=precision)
+    ch.upload(var)
+    rc = ch.step(warmup)
+    if rc:
+        raise RuntimeError(f"{precision}: blow-up at warm-up step {rc}")
+    if profile:
+        ch.profile(1)
+    t0 = time.perf_counter()
+    rc = ch.step(steps)          # returns after a device synchronisation
+    t1 = time.perf_counter()
+    if rc:
+        raise RuntimeError(f"{precision}: blow-up at step {rc}")
+    kms, kn = ch.profile(0) if profile else (0.0, 0)
+    tau = ch.stats()[0]
+    ch.close()
+    return (t1 - t0), kms, kn, tau
+
+
+def cpu_baseline(var, nsteps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the CPU restatement, timed as the baseline only
+
+    n1, n2, n3 = NP
+    o = orc.Oracle(n1, n2, n3, dt=DT, istat=ISTAT)
+    o.load(var)
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    t1 = time.perf_counter()
+    rate = n1 * n2 * n3 * 2 * nsteps / (t1 - t0)
+    return rate, t1 - t0
+
+
+def traffic_from_profiles(precision):
+    path = os.path.join(ROOT, "profiles", f"traffic_stage_{precision}_256x128x192.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64-strict", "mixed"])
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="steps of the CPU restatement timed for cpu_baseline (0 = skip)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one channel instance per GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    n1, n2, n3 = NP
+    nodes = n1 * n2 * n3
+    log(f"[bench] seeded IC nP={NP}")
+    var = formats.laminar_ic(n1, n2, n3)
+
+    log(f"[bench] {args.precision}: warmup {args.warmup}, timed {args.steps} steps")
+    t, kms, kn, tau = run_gpu(var, args.precision, args.steps, args.warmup, profile=True)
+    value = nodes * 2 * args.steps / t
+    ms_step = 1000.0 * t / args.steps
+    avg_launch_s = (kms / kn) / 1000.0 if kn else float("nan")
+    achieved = ALG_BYTES_PER_NODE_STAGE * nodes / avg_launch_s / 1e9
+    log(f"[bench] {args.precision}: {ms_step:.4f} ms/step, {value:.4e} cell-updates/s/stage, "
+        f"stage kernel {1000 * avg_launch_s:.4f} ms avg over {kn} launches, tau={tau}")
+
+    mixed = None
+    if not args.no_mixed and args.precision != "mixed":
+        tm, kmm, knm, taum = run_gpu(var, "mixed", args.steps, args.warmup, profile=True)
+        vm = nodes * 2 * args.steps / tm
+        mixed = {"value": vm, "ms_per_step": 1000.0 * tm / args.steps,
+                 "ratio_vs_fp64": vm / value,
+                 "stage_kernel_ms": (kmm / knm) if knm else None}
+        log(f"[bench] mixed: {1000 * tm / args.steps:.4f} ms/step, ratio {vm / value:.3f}")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps on one core")
+        rate, tc = cpu_baseline(var, args.cpu_steps)
+        cpu = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of nP={list(NP)} FP64 (C restatement, "
+                         f"{tc:.1f} s)"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision != "mixed" else "f32/f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "channel DNS time step, 256x192x128 (reference nP=[256,128,192]), "
+                        "Ma 1.5, Re 3000, RK2, hybrid central/QUICK, seeded perturbed laminar start",
+            "nP": list(NP), "dt": DT, "istat": ISTAT, "precision": args.precision,
+            "mixed": mixed,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(args.precision),
+            "kernel": "k_stage_fused" if args.precision != "fp64-strict" else "k_rhs_strict",
+            "avg_launch_ms": 1000 * avg_launch_s,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""Python host side of the MI355X channel time step: a ctypes binding of the
+C ABI in include/cc_abi.h (libcc_hip.so, built in-tree).
+
+The binding mirrors the reference's step seam (v1.10/codes/main_v1.10.f90:
+37-53): upload the state read from a solution file, advance it, read the
+statistics that the reference appends to tauWall.dat / moment*.dat, and
+download var(n1,n2,n3,7) for the solution writer.  It refuses to run when
+the HIP library is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcc_hip.so")
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include", "cc_abi.h")
+
+FP64_STRICT, FP64, MIXED = 0, 1, 2
+PRECISIONS = {"fp64-strict": FP64_STRICT, "fp64": FP64, "mixed": MIXED}
+
+# Every function the C ABI exports (include/cc_abi.h).
+ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
+               "cc_blown", "cc_profile", "cc_destroy", "cc_last_error", "cc_abi_version")
+
+
+class cc_params(C.Structure):
+    _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int), ("istat", C.c_int),
+                ("precision", C.c_int), ("reserved", C.c_int),
+                ("Re", C.c_double), ("Mac", C.c_double), ("dt", C.c_double),
+                ("PexVal", C.c_double), ("PeyVal", C.c_double), ("PezVal", C.c_double),
+                ("Ri_b", C.c_double), ("slip_len", C.c_double)]
+
+
+class CCError(RuntimeError):
+    pass
+
+
+def build() -> str:
+    """Compile libcc_hip.so for gfx950 (hipcc, in-tree)."""
+    subprocess.run(["make", "-s", "-C", CSRC], check=True)
+    return LIB_PATH
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def load_library():
+    """Load libcc_hip.so; raise if it is not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CCError(f"{LIB_PATH} is not built; run compchan3dmpi_amd.channel.build()")
+    L = C.CDLL(LIB_PATH)
+    L.cc_create.argtypes = [C.POINTER(cc_params), C.POINTER(C.c_void_p)]
+    L.cc_upload.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double]
+    L.cc_step.argtypes = [C.c_void_p, C.c_int]
+    L.cc_stats.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.cc_diag.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+    L.cc_download.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.cc_blown.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.cc_profile.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.cc_destroy.argtypes = [C.c_void_p]
+    L.cc_destroy.restype = None
+    L.cc_last_error.restype = C.c_char_p
+    for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
+              "cc_blown", "cc_profile", "cc_abi_version"):
+        getattr(L, f).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(L, rc):
+    if rc < 0:
+        raise CCError(f"cc error {rc}: {L.cc_last_error().decode()}")
+    return rc
+
+
+class Channel:
+    """One channel instance on the current HIP device.
+
+    Arrays use the reference's Fortran order exposed as C-order numpy arrays:
+    var has shape (7, n3, n2, n1); stat1/stat2 (7, n3); corr (9, n3)."""
+
+    def __init__(self, n1, n2, n3, Re=3000.0, Mac=1.5, dt=1e-4, istat=10, PexVal=2.0,
+                 PeyVal=2.0, PezVal=0.2, precision="fp64", Ri_b=0.0, slip_len=0.0):
+        self.L = load_library()
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.p = cc_params(n1, n2, n3, istat, prec, 0, Re, Mac, dt, PexVal, PeyVal, PezVal, Ri_b,
+                           slip_len)
+        self.n = (n1, n2, n3)
+        h = C.c_void_p()
+        _check(self.L, self.L.cc_create(C.byref(self.p), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def shape(self):
+        n1, n2, n3 = self.n
+        return (n3, n2, n1)
+
+    def upload(self, var, nstep=0, time=0.0):
+        var = np.ascontiguousarray(var, dtype=np.float64)
+        if var.shape != (7,) + self.shape:
+            raise ValueError(f"var has shape {var.shape}, expected {(7,) + self.shape}")
+        _check(self.L, self.L.cc_upload(self.h, var, nstep, time))
+
+    def step(self, nsteps=1):
+        """Advance nsteps; returns 0 or the step at which the blow-up check fired."""
+        return _check(self.L, self.L.cc_step(self.h, nsteps))
+
+    def stats(self):
+        n3 = self.n[2]
+        tau = np.empty(2)
+        s1 = np.empty((7, n3))
+        s2 = np.empty((7, n3))
+        cr = np.empty((9, n3))
+        _check(self.L, self.L.cc_stats(self.h, tau, s1, s2, cr))
+        return tau, s1, s2, cr
+
+    def diag(self, x3_file, x2_end):
+        out = np.empty(16)
+        _check(self.L, self.L.cc_diag(self.h, np.ascontiguousarray(x3_file, dtype=np.float64),
+                                      float(x2_end), out))
+        return out
+
+    def download(self):
+        out = np.empty((7,) + self.shape)
+        ns = C.c_int()
+        t = C.c_double()
+        _check(self.L, self.L.cc_download(self.h, out, C.byref(ns), C.byref(t)))
+        return out, ns.value, t.value
+
+    def profile(self, mode=2):
+        """mode 1: reset and enable per-launch HIP-event timing of the stage
+        kernel; 0: disable; 2: query.  Returns (total_ms, launches)."""
+        out = np.zeros(2)
+        _check(self.L, self.L.cc_profile(self.h, mode, out))
+        return float(out[0]), int(out[1])
+
+    def blown(self):
+        ijk = (C.c_int * 3)()
+        step = self.L.cc_blown(self.h, ijk)
+        return step, tuple(ijk)

@@ -1,0 +1,366 @@
+// cc_abi.hip -- C ABI over the HIP kernels (declared in include/cc_abi.h).
+//
+// Host-side work here is set-up and marshalling only: the mesh and one-sided
+// coefficients (the reference's own host set-up, comdata.f90:70-134 and
+// main_v1.9.f90:52-76), buffer management, the step loop that enqueues
+// kernels, and copies.  Every per-node computation runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cc_abi.h"
+#include "cc_device.h"
+#include "cc_kernels.h"
+
+using namespace cc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return fail(CC_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+struct Chan {
+  cc_params prm{};
+  Geom g{};
+  Phys ph{};
+  int n1 = 0, n2 = 0, n3 = 0;
+  hipStream_t s = nullptr;
+  double *UA = nullptr, *UB = nullptr, *W = nullptr, *rw = nullptr, *pw = nullptr;
+  double *jac = nullptr, *invJac = nullptr, *force = nullptr, *consts = nullptr;
+  double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr;
+  int* halt = nullptr;
+  unsigned long long* blow = nullptr;
+  int nstep = 0;
+  double time = 0.0;
+  bool loaded = false, fresh = true;
+  int blown_step = 0;
+  int blown_ijk[3] = {0, 0, 0};
+  std::vector<double> x3;
+  FusedCfg cfg{32, 16, 0};
+  // per-launch timing of the stage's main kernel (cc_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms = 0.0;
+  long long prof_n = 0;
+
+  WallP wallp() const {
+    if (prm.precision == CC_FP64_STRICT) return WallP{W + 5 * g.field, (long long)(n3 - 1) * g.plane};
+    return WallP{pw, g.plane};
+  }
+};
+
+// Mesh and coefficients in the reference's own arithmetic (comdata.f90:74-110,
+// main_v1.9.f90:59-83).
+void setup_mesh(Chan& c) {
+  const int n1 = c.n1, n2 = c.n2, n3 = c.n3;
+  Phys& ph = c.ph;
+  const double gama = 1.4, one = 1.0, haf = 0.5;
+  const double Pi = 4.0 * std::atan(1.0);
+  const double x1e = 4.0 * Pi, x2e = 4.0 * Pi / 3.0, x3e = 2.0;
+  ph.dx = x1e / (double)(n1 - 1);
+  ph.dy = x2e / (double)(n2 - 1);
+  ph.dXi = x3e / (double)(n3 - 1);
+  ph.invdx = 1.0 / ph.dx;
+  ph.invdy = 1.0 / ph.dy;
+  ph.invdXi = 1.0 / ph.dXi;
+  std::vector<double> Xi(n3, 0.0), jac(n3), invJac(n3, 0.0);
+  c.x3.assign(n3, 0.0);
+  for (int k = 0; k < n3 - 1; k++) Xi[k + 1] = Xi[k] + ph.dXi;
+  for (int k = 0; k < n3; k++) {
+    c.x3[k] = 1.0 - std::cos(0.50 * Xi[k] * Pi);
+    jac[k] = 0.50 * Pi * std::sin(0.50 * Xi[k] * Pi);
+  }
+  for (int k = 1; k < n3; k++) invJac[k] = 1.0 / jac[k];
+  const std::vector<double>& x3 = c.x3;
+  double dz1 = x3[1] - x3[0];
+  double alpha = (x3[2] - x3[1]) / dz1;
+  double t = dz1 * alpha * (alpha + one);
+  ph.f1d2c0 = (one - (alpha + one) * (alpha + one)) / t;
+  ph.f1d2c1 = ((alpha + one) * (alpha + one)) / t;
+  ph.f1d2c2 = -one / t;
+  t = haf * (dz1 * dz1) * alpha * (alpha + one);
+  ph.f2d1c0 = alpha / t;
+  ph.f2d1c1 = -(alpha + one) / t;
+  ph.f2d1c2 = one / t;
+  double dzN = x3[n3 - 1] - x3[n3 - 2];
+  alpha = (x3[n3 - 2] - x3[n3 - 3]) / dzN;
+  t = dzN * alpha * (alpha + one);
+  ph.b1d2c0 = ((alpha + one) * (alpha + one) - one) / t;
+  ph.b1d2c1 = -((alpha + one) * (alpha + one)) / t;
+  ph.b1d2c2 = one / t;
+  t = haf * (dzN * dzN) * alpha * (alpha + one);
+  ph.b2d1c0 = alpha / t;
+  ph.b2d1c1 = -(alpha + one) / t;
+  ph.b2d1c2 = one / t;
+  ph.dz1 = dz1;
+  ph.dzN = dzN;
+  ph.Re = c.prm.Re;
+  ph.invRe = 1.0 / c.prm.Re;
+  ph.ggM = gama * (gama - 1) * (c.prm.Mac * c.prm.Mac);
+  ph.gM2 = gama * (c.prm.Mac * c.prm.Mac);
+  ph.dt = c.prm.dt;
+  ph.Pex = c.prm.PexVal;
+  ph.Pey = c.prm.PeyVal;
+  ph.Pez = c.prm.PezVal;
+  ph.gkRP = gama * ph.invRe * kInvPr;
+  (void)hipMemcpy(c.jac, jac.data(), n3 * sizeof(double), hipMemcpyHostToDevice);
+  (void)hipMemcpy(c.invJac, invJac.data(), n3 * sizeof(double), hipMemcpyHostToDevice);
+}
+
+void free_chan(Chan* c) {
+  if (!c) return;
+  double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->jac, &c->invJac, &c->force,
+                     &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red};
+  for (double** b : bufs)
+    if (*b) (void)hipFree(*b);
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  if (c->halt) (void)hipFree(c->halt);
+  if (c->blow) (void)hipFree(c->blow);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cc_abi_version(void) { return CC_ABI_VERSION; }
+const char* cc_last_error(void) { return g_err.c_str(); }
+
+int cc_create(const cc_params* p, void** handle) {
+  if (!p || !handle) return fail(CC_EINVAL, "null argument");
+  *handle = nullptr;
+  if (p->n1 < 6 || p->n2 < 6 || p->n3 < 6)
+    return fail(CC_EINVAL, "grid too small: every nP must be >= 6");
+  if (p->istat < 1) return fail(CC_EINVAL, "istat must be >= 1");
+  if (!(p->dt > 0.0) || !(p->Re > 0.0) || !(p->Mac > 0.0))
+    return fail(CC_EINVAL, "dt, Re and Mac must be positive");
+  if (p->precision < CC_FP64_STRICT || p->precision > CC_MIXED)
+    return fail(CC_EINVAL, "unknown precision mode");
+  if (p->Ri_b != 0.0 || p->slip_len != 0.0)
+    if (p->precision == CC_FP64_STRICT)
+      return fail(CC_EINVAL, "buoyancy and wall slip are not part of the reference-order path");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(CC_EHIP, "no HIP device");
+  Chan* c = new Chan();
+  c->prm = *p;
+  c->n1 = p->n1;
+  c->n2 = p->n2;
+  c->n3 = p->n3;
+  c->g.P1 = p->n1 - 2;
+  c->g.P2 = p->n2 - 2;
+  c->g.n3 = p->n3;
+  c->g.plane = (long long)c->g.P1 * c->g.P2;
+  c->g.field = c->g.plane * p->n3;
+  const size_t F = (size_t)c->g.field, PL = (size_t)c->g.plane, n3 = p->n3;
+  const size_t full = (size_t)p->n1 * p->n2 * p->n3;
+  auto al = [&](double** b, size_t n) { return hipMalloc((void**)b, n * sizeof(double)); };
+  if (hipStreamCreate(&c->s) != hipSuccess || al(&c->UA, 5 * F) != hipSuccess ||
+      al(&c->UB, 5 * F) != hipSuccess || al(&c->rw, 2 * PL) != hipSuccess ||
+      al(&c->pw, 2 * PL) != hipSuccess || al(&c->jac, n3) != hipSuccess ||
+      al(&c->invJac, n3) != hipSuccess || al(&c->force, 2 + n3) != hipSuccess ||
+      al(&c->consts, kNStat * n3) != hipSuccess || al(&c->stats, kNStat * n3 + 2) != hipSuccess ||
+      al(&c->Vfile, 7 * full) != hipSuccess || al(&c->red, 4096) != hipSuccess ||
+      hipMalloc((void**)&c->halt, sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->blow, sizeof(unsigned long long)) != hipSuccess ||
+      (p->precision == CC_FP64_STRICT && al(&c->W, 7 * F) != hipSuccess)) {
+    free_chan(c);
+    return fail(CC_ENOMEM, "device allocation failed");
+  }
+  setup_mesh(*c);
+  if (hipMemset(c->force, 0, (2 + n3) * sizeof(double)) != hipSuccess ||
+      hipMemset(c->stats, 0, (kNStat * n3 + 2) * sizeof(double)) != hipSuccess) {
+    free_chan(c);
+    return fail(CC_EHIP, "device memset failed");
+  }
+  *handle = c;
+  return 0;
+}
+
+int cc_upload(void* handle, const double* var, int nstep, double time) {
+  Chan* c = (Chan*)handle;
+  if (!c || !var) return fail(CC_EINVAL, "null argument");
+  const size_t full = (size_t)c->n1 * c->n2 * c->n3;
+  HIPCHK(hipMemcpyAsync(c->Vfile, var, 7 * full * sizeof(double), hipMemcpyHostToDevice, c->s));
+  launch_upload(c->g, c->ph, c->n1, c->n2, c->Vfile, c->UA, c->UB, c->W, c->wallp(), c->rw,
+                c->consts, c->s);
+  HIPCHK(hipGetLastError());
+  // forcing state: zero until the first statistics step (the reference's
+  // static initialisation of tauxzL, tauxzU, stat1Spc)
+  HIPCHK(hipMemsetAsync(c->force, 0, (2 + c->n3) * sizeof(double), c->s));
+  HIPCHK(hipMemsetAsync(c->halt, 0, sizeof(int), c->s));
+  const unsigned long long big = ~0ull;
+  HIPCHK(hipMemcpyAsync(c->blow, &big, sizeof(big), hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->nstep = nstep;
+  c->time = time;
+  c->loaded = true;
+  c->fresh = true;
+  c->blown_step = 0;
+  std::memset(c->blown_ijk, 0, sizeof c->blown_ijk);
+  return 0;
+}
+
+int cc_step(void* handle, int nsteps) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  if (!c->loaded) return fail(CC_ESTATE, "no state uploaded");
+  if (nsteps < 0) return fail(CC_EINVAL, "negative step count");
+  if (c->blown_step) return c->blown_step;
+  const int step0 = c->nstep;
+  const double time0 = c->time;
+  if (c->prof && c->evpool.size() < (size_t)4 * nsteps) {
+    const size_t need = (size_t)4 * nsteps;
+    while (c->evpool.size() < need) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->evpool.push_back(e);
+    }
+  }
+  auto ev = [&](int s, int k) -> hipEvent_t { return c->prof ? c->evpool[(size_t)4 * s + k] : nullptr; };
+  for (int s = 0; s < nsteps; s++) {
+    const int step = c->nstep + 1;
+    if (c->prm.precision == CC_FP64_STRICT) {
+      launch_stage_strict(c->g, c->ph, c->UA, c->UA, c->UB, c->W, c->rw, c->jac, c->invJac,
+                          c->force, 1, c->halt, c->blow, step, c->n1, c->n2, c->s, ev(s, 0),
+                          ev(s, 1));
+      launch_stage_strict(c->g, c->ph, c->UB, c->UA, c->UA, c->W, c->rw, c->jac, c->invJac,
+                          c->force, 2, c->halt, c->blow, step, c->n1, c->n2, c->s, ev(s, 2),
+                          ev(s, 3));
+    } else {
+#ifndef CC_HAVE_FUSED
+      return fail(CC_EINVAL, "fused precision modes are not built into this library");
+#else
+      const int mixed = c->prm.precision == CC_MIXED;
+      launch_stage_fused(c->g, c->ph, mixed, c->cfg, c->UA, c->UA, c->UB, c->pw, c->rw, c->jac,
+                         c->invJac, c->force, 1, c->halt, c->blow, step, c->n1, c->n2, c->s,
+                         ev(s, 0), ev(s, 1));
+      launch_stage_fused(c->g, c->ph, mixed, c->cfg, c->UB, c->UA, c->UA, c->pw, c->rw, c->jac,
+                         c->invJac, c->force, 2, c->halt, c->blow, step, c->n1, c->n2, c->s,
+                         ev(s, 2), ev(s, 3));
+#endif
+    }
+    if (step % c->prm.istat == 0)
+      launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
+                   c->stats, c->halt, step, c->s);
+    c->nstep = step;
+    c->time = c->time + c->prm.dt;
+  }
+  HIPCHK(hipGetLastError());
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, c->halt, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (nsteps > 0) c->fresh = false;
+  if (c->prof)
+    for (int s = 0; s < nsteps; s++)
+      for (int k = 0; k < 2; k++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev(s, 2 * k), ev(s, 2 * k + 1)));
+        c->prof_ms += ms;
+        c->prof_n += 1;
+      }
+  if (h) {
+    unsigned long long lin = 0;
+    HIPCHK(hipMemcpy(&lin, c->blow, sizeof(lin), hipMemcpyDeviceToHost));
+    c->blown_step = h;
+    c->blown_ijk[0] = (int)(lin % c->n1) + 1;
+    c->blown_ijk[1] = (int)((lin / c->n1) % c->n2) + 1;
+    c->blown_ijk[2] = (int)(lin / ((unsigned long long)c->n1 * c->n2)) + 1;
+    // the state is that of step h: rewind the counters the same way
+    c->nstep = step0;
+    c->time = time0;
+    while (c->nstep < h) {
+      c->nstep++;
+      c->time = c->time + c->prm.dt;
+    }
+    return h;
+  }
+  return 0;
+}
+
+int cc_stats(void* handle, double* tau, double* stat1, double* stat2, double* corr) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  const int n3 = c->n3;
+  std::vector<double> h((size_t)kNStat * n3 + 2);
+  HIPCHK(hipMemcpyAsync(h.data(), c->stats, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (tau) {
+    tau[0] = h[(size_t)kNStat * n3];
+    tau[1] = h[(size_t)kNStat * n3 + 1];
+  }
+  if (stat1) std::memcpy(stat1, h.data(), 7 * n3 * sizeof(double));
+  if (stat2) std::memcpy(stat2, h.data() + 7 * n3, 7 * n3 * sizeof(double));
+  if (corr) std::memcpy(corr, h.data() + 14 * n3, 9 * n3 * sizeof(double));
+  return 0;
+}
+
+int cc_download(void* handle, double* var, int* nstep, double* time) {
+  Chan* c = (Chan*)handle;
+  if (!c || !var) return fail(CC_EINVAL, "null argument");
+  if (!c->loaded) return fail(CC_ESTATE, "no state uploaded");
+  const size_t full = (size_t)c->n1 * c->n2 * c->n3;
+  if (c->fresh) {
+    HIPCHK(hipMemcpyAsync(var, c->Vfile, 7 * full * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  } else {
+    if (!c->Vdev) HIPCHK(hipMalloc((void**)&c->Vdev, 7 * full * sizeof(double)));
+    launch_download(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->Vfile, c->Vdev, c->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(var, c->Vdev, 7 * full * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (nstep) *nstep = c->nstep;
+  if (time) *time = c->time;
+  return 0;
+}
+
+int cc_blown(void* handle, int* ijk) {
+  Chan* c = (Chan*)handle;
+  if (!c || !ijk) return fail(CC_EINVAL, "null argument");
+  std::memcpy(ijk, c->blown_ijk, sizeof c->blown_ijk);
+  return c->blown_step;
+}
+
+int cc_profile(void* handle, int mode, double* out) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  if (mode == 1) {
+    c->prof = true;
+    c->prof_ms = 0.0;
+    c->prof_n = 0;
+  } else if (mode == 0) {
+    c->prof = false;
+  }
+  if (out) {
+    out[0] = c->prof_ms;
+    out[1] = (double)c->prof_n;
+  }
+  return 0;
+}
+
+int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
+  Chan* c = (Chan*)handle;
+  if (!c || !x3_file || !out) return fail(CC_EINVAL, "null argument");
+  (void)x2_end;
+  return fail(CC_EINVAL, "bulk diagnostics are not built into this library");
+}
+
+void cc_destroy(void* handle) { free_chan((Chan*)handle); }
+
+}  // extern "C"

@@ -1,0 +1,308 @@
+// cc_common.hip -- state upload/download and the FP64 reductions that feed
+// the bulk forcing (wall shear and plane-mean density), shared by every
+// precision mode.  Compiled with -ffp-contract=off: the per-node primitive
+// recovery here must give the same bits as the stage kernels' own.
+#include "cc_device.h"
+#include "cc_kernels.h"
+
+namespace cc {
+
+// Fortran var(i,j,k,c), 0-based indices.
+__device__ __forceinline__ long long fidx(int n1, int n2, int n3, int i, int j, int k, int c) {
+  return (((long long)c * n3 + k) * n2 + j) * n1 + i;
+}
+
+// Primitives of a conserved state (main_v1.9.f90:599-609).
+struct Prim {
+  double rho, u, v, w, E, T, p;
+};
+__device__ __forceinline__ Prim prim_of(const Phys& ph, double r, double ru, double rv, double rw,
+                                        double rE) {
+  Prim P;
+  P.rho = r;
+  P.u = ru / r;
+  P.v = rv / r;
+  P.w = rw / r;
+  P.E = rE / r;
+  P.T = P.E - ph.ggM * 0.5 * (P.u * P.u + P.v * P.v + P.w * P.w);
+  P.p = P.rho * P.T / ph.gM2;
+  return P;
+}
+
+// The 23 plane-statistics quantities of main_v1.9.f90:993-1009.
+__device__ __forceinline__ void stat_terms(const Prim& P, double q[kNStat]) {
+  const double v[7] = {P.rho, P.u, P.v, P.w, P.E, P.T, P.p};
+  for (int c = 0; c < 7; c++) {
+    q[c] = v[c];
+    q[7 + c] = v[c] * v[c];
+  }
+  q[14] = P.rho * P.w;
+  q[15] = P.rho * P.T;
+  q[16] = P.u * P.v;
+  q[17] = P.u * P.w;
+  q[18] = P.v * P.w;
+  q[19] = P.v * P.T;
+  q[20] = P.T * P.w;
+  q[21] = P.p * P.w;
+  q[22] = P.p * P.T;
+}
+
+// Wall state after the boundary update (main_v1.9.f90:624-639): u=v=w=0,
+// T=1, E=T, rho = gama Ma^2 p / T.
+__device__ __forceinline__ Prim wall_prim(const Phys& ph, double rho, double p) {
+  Prim P;
+  P.rho = rho;
+  P.u = 0.0;
+  P.v = 0.0;
+  P.w = 0.0;
+  P.T = 1.0;
+  P.E = 1.0 + ph.ggM * 0.5 * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+  P.p = p;
+  return P;
+}
+
+// ---- upload: conserved state from var (main_v1.9.f90:284-294) -----------
+__global__ void k_upload(Geom g, Phys ph, int n1, int n2, const double* __restrict__ V,
+                         double* __restrict__ UA, double* __restrict__ UB, double* __restrict__ W,
+                         WallP pw, double* __restrict__ rw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.field) return;
+  const int ii = (int)(t % g.P1);
+  const int jj = (int)((t / g.P1) % g.P2);
+  const int kk = (int)(t / g.plane);
+  const int n3 = g.n3;
+  double v[7];
+  for (int c = 0; c < 7; c++) v[c] = V[fidx(n1, n2, n3, ii + 1, jj + 1, kk, c)];
+  const long long F = g.field;
+  UA[t] = v[0];
+  UB[t] = v[0];
+  for (int c = 1; c < 5; c++) {
+    UA[c * F + t] = v[0] * v[c];
+    UB[c * F + t] = v[0] * v[c];
+  }
+  if (W) {  // reference-order path keeps the file's primitives and mu = T**0.7
+    W[0 * F + t] = v[1];
+    W[1 * F + t] = v[2];
+    W[2 * F + t] = v[3];
+    W[3 * F + t] = v[4];
+    W[4 * F + t] = v[5];
+    W[5 * F + t] = v[6];
+    W[6 * F + t] = pow(v[5], 0.7);
+  }
+  const long long n2d = (long long)jj * g.P1 + ii;
+  if (kk == 0) {
+    pw.p[n2d] = v[6];
+    rw[n2d] = v[0];
+  } else if (kk == n3 - 1) {
+    pw.p[pw.top_off + n2d] = v[6];
+    rw[g.plane + n2d] = v[0];
+  }
+}
+
+// Constant part of the plane sums: nodes the reference never refreshes.
+// Interior planes: the four corner columns (i,j) in {1,n1}x{1,n2}
+// (main_v1.9.f90:589-596 never writes them).  Wall planes: the whole halo
+// ring i in {1,n1} or j in {1,n2}, whose pressure is never recomputed
+// (:645 loops over updated i,j only) and whose other values are set by the
+// wall update (:624-639).
+__global__ void k_consts(Geom g, Phys ph, int n1, int n2, const double* __restrict__ V,
+                         double* __restrict__ C) {
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n3 = g.n3;
+  if (kk >= n3) return;
+  double acc[kNStat];
+  for (int q = 0; q < kNStat; q++) acc[q] = 0.0;
+  double tq[kNStat];
+  if (kk > 0 && kk < n3 - 1) {
+    const int ci[2] = {0, n1 - 1}, cj[2] = {0, n2 - 1};
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        Prim P;
+        P.rho = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 0)];
+        P.u = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 1)];
+        P.v = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 2)];
+        P.w = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 3)];
+        P.E = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 4)];
+        P.T = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 5)];
+        P.p = V[fidx(n1, n2, n3, ci[a], cj[b], kk, 6)];
+        stat_terms(P, tq);
+        for (int q = 0; q < kNStat; q++) acc[q] += tq[q];
+      }
+  } else {
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        if (!(i == 0 || i == n1 - 1 || j == 0 || j == n2 - 1)) continue;
+        const double p = V[fidx(n1, n2, n3, i, j, kk, 6)];
+        Prim P = wall_prim(ph, ph.gM2 * p / 1.0, p);
+        stat_terms(P, tq);
+        for (int q = 0; q < kNStat; q++) acc[q] += tq[q];
+      }
+  }
+  for (int q = 0; q < kNStat; q++) C[q * n3 + kk] = acc[q];
+}
+
+// ---- download: the reference's var(n1,n2,n3,7) with every halo value ----
+__global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __restrict__ U,
+                           WallP pw, const double* __restrict__ rw, const double* __restrict__ V,
+                           double* __restrict__ Vout) {
+  const long long total = (long long)n1 * n2 * g.n3;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int i = (int)(t % n1);
+  const int j = (int)((t / n1) % n2);
+  const int k = (int)(t / ((long long)n1 * n2));
+  const int n3 = g.n3;
+  const bool ei = (i == 0 || i == n1 - 1), ej = (j == 0 || j == n2 - 1);
+  const bool wall = (k == 0 || k == n3 - 1);
+  Prim P;
+  if (!wall && ei && ej) {  // stale corner column
+    for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, n3, i, j, k, c)] = V[fidx(n1, n2, n3, i, j, k, c)];
+    return;
+  }
+  if (wall && (ei || ej)) {
+    const double p = V[fidx(n1, n2, n3, i, j, k, 6)];
+    P = wall_prim(ph, ph.gM2 * p / 1.0, p);
+  } else {
+    const int ii = (i == 0) ? g.P1 - 1 : (i == n1 - 1 ? 0 : i - 1);
+    const int jj = (j == 0) ? g.P2 - 1 : (j == n2 - 1 ? 0 : j - 1);
+    const long long n2d = (long long)jj * g.P1 + ii;
+    if (wall) {
+      const bool top = (k == n3 - 1);
+      P = wall_prim(ph, rw[(top ? g.plane : 0) + n2d], pw.p[(top ? pw.top_off : 0) + n2d]);
+    } else {
+      const long long F = g.field, n = idx(g, ii, jj, k);
+      P = prim_of(ph, U[n], U[F + n], U[2 * F + n], U[3 * F + n], U[4 * F + n]);
+    }
+  }
+  const double v[7] = {P.rho, P.u, P.v, P.w, P.E, P.T, P.p};
+  for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, n3, i, j, k, c)] = v[c];
+}
+
+// ---- deterministic block reduction ---------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&a)[NV], double* lds /* [nwaves][NV] */) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < NV; q++) {
+    double x = a[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    a[q] = x;
+  }
+  if (lane == 0)
+    for (int q = 0; q < NV; q++) lds[wave * NV + q] = a[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int q = 0; q < NV; q++) {
+      double s = lds[q];
+      for (int w = 1; w < nw; w++) s += lds[w * NV + q];
+      a[q] = s;
+    }
+  }
+}
+
+// Wall shear (main_v1.9.f90:929-988): block 0 bottom, block 1 top.
+__global__ void __launch_bounds__(256) k_tauwall(Geom g, Phys ph, int n1, int n2,
+                                                 const double* __restrict__ U,
+                                                 double* __restrict__ force,
+                                                 double* __restrict__ tau_out, const int* halt,
+                                                 int step) {
+  __shared__ double lds[4];
+  const int h = *halt;
+  if (h != 0 && h <= step) return;
+  const bool top = blockIdx.x == 1;
+  const long long F = g.field;
+  const int N = g.n3 - 1;
+  double acc[1] = {0.0};
+  for (long long n = threadIdx.x; n < g.plane; n += blockDim.x) {
+    const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
+    // wall values after the boundary update: u = w = 0, mu = 1**0.7
+    const double mu = pow(1.0, 0.7);
+    const double dwdx = mu * (0.0 - 0.0) * 0.5 * ph.invdx;
+    const long long a = idx(g, ii, jj, top ? N - 1 : 1), b = idx(g, ii, jj, top ? N - 2 : 2);
+    const double u2 = U[F + a] / U[a], u3 = U[F + b] / U[b];
+    double dudz;
+    if (!top)
+      dudz = mu * (ph.f1d2c0 * 0.0 + ph.f1d2c1 * u2 + ph.f1d2c2 * u3);
+    else
+      dudz = mu * (ph.b1d2c0 * 0.0 + ph.b1d2c1 * u2 + ph.b1d2c2 * u3);
+    acc[0] += (dwdx + dudz) * ph.invRe;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) {
+    const double t = acc[0] / (double)((n1 - 2) * (n2 - 2));
+    force[top ? 1 : 0] = t;
+    tau_out[top ? 1 : 0] = t;
+  }
+}
+
+// Plane statistics (main_v1.9.f90:993-1025): one block per plane.  Each
+// unique interior node is weighted by the number of times the reference's
+// all-node sum sees it (1 + an x duplicate + a y duplicate); the stale
+// corners and wall rings come in through C.
+__global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int n2,
+                                                    const double* __restrict__ U, WallP pw,
+                                                    const double* __restrict__ rw,
+                                                    const double* __restrict__ C,
+                                                    double* __restrict__ force,
+                                                    double* __restrict__ stats, const int* halt,
+                                                    int step) {
+  __shared__ double lds[4 * kNStat];
+  const int h = *halt;
+  if (h != 0 && h <= step) return;
+  const int kk = blockIdx.x, n3 = g.n3;
+  const bool wall = (kk == 0 || kk == n3 - 1);
+  const long long F = g.field;
+  double acc[kNStat];
+  for (int q = 0; q < kNStat; q++) acc[q] = 0.0;
+  double tq[kNStat];
+  for (long long n = threadIdx.x; n < g.plane; n += blockDim.x) {
+    const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
+    Prim P;
+    double wgt = 1.0;
+    if (wall) {
+      const bool top = kk == n3 - 1;
+      P = wall_prim(ph, rw[(top ? g.plane : 0) + n], pw.p[(top ? pw.top_off : 0) + n]);
+    } else {
+      const long long m = idx(g, ii, jj, kk);
+      P = prim_of(ph, U[m], U[F + m], U[2 * F + m], U[3 * F + m], U[4 * F + m]);
+      wgt = 1.0 + ((ii == 0 || ii == g.P1 - 1) ? 1.0 : 0.0) + ((jj == 0 || jj == g.P2 - 1) ? 1.0 : 0.0);
+    }
+    stat_terms(P, tq);
+    for (int q = 0; q < kNStat; q++) acc[q] += wgt * tq[q];
+  }
+  block_sum<kNStat>(acc, lds);
+  if (threadIdx.x == 0) {
+    const double nXY = (double)(n1 * n2);
+    for (int q = 0; q < kNStat; q++) stats[q * n3 + kk] = (acc[q] + C[q * n3 + kk]) / nXY;
+    force[2 + kk] = (acc[0] + C[kk]) / nXY;
+  }
+}
+
+void launch_upload(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull, double* UA,
+                   double* UB, double* W, WallP pw, double* rw, double* consts, hipStream_t s) {
+  const int B = 256;
+  hipLaunchKernelGGL(k_upload, dim3((unsigned)((g.field + B - 1) / B)), dim3(B), 0, s, g, ph, n1,
+                     n2, Vfull, UA, UB, W, pw, rw);
+  hipLaunchKernelGGL(k_consts, dim3((g.n3 + 63) / 64), dim3(64), 0, s, g, ph, n1, n2, Vfull,
+                     consts);
+}
+
+void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
+                     const double* rw, const double* Vfull, double* Vout, hipStream_t s) {
+  const long long total = (long long)n1 * n2 * g.n3;
+  const int B = 256;
+  hipLaunchKernelGGL(k_download, dim3((unsigned)((total + B - 1) / B)), dim3(B), 0, s, g, ph, n1,
+                     n2, U, pw, rw, Vfull, Vout);
+}
+
+void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
+                  const double* rw, const double* consts, double* force, double* stats,
+                  const int* halt, int step, hipStream_t s) {
+  // stats layout: [23][n3] plane statistics, then tau[2]
+  hipLaunchKernelGGL(k_tauwall, dim3(2), dim3(256), 0, s, g, ph, n1, n2, U, force,
+                     stats + kNStat * g.n3, halt, step);
+  hipLaunchKernelGGL(k_planestats, dim3(g.n3), dim3(256), 0, s, g, ph, n1, n2, U, pw, rw, consts,
+                     force, stats, halt, step);
+}
+
+}  // namespace cc

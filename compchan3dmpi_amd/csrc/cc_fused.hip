@@ -1,0 +1,688 @@
+// cc_fused.hip -- one fused kernel per RK stage: primitive recovery, hybrid
+// central/QUICK convective flux, viscous/pressure point fluxes and their
+// divergence, forcing source and RK update, streaming along the wall-normal
+// direction (2.5-D blocking).
+//
+// Math is the reference's (v1.10/codes/main_v1.9.f90:329-609); the
+// evaluation order is free (FMA contraction allowed), so FP64 results agree
+// with the reference-order path to rounding, not bit for bit.  With
+// T = float the flux and stencil arithmetic is FP32 while the conserved
+// state, the RK accumulation and every reduction stay FP64.
+//
+// Work decomposition.  A block owns a TX x TY tile of (ii, jj) columns and a
+// chunk of output planes [kb, ke).  It walks planes t = kb-2 .. ke+1; at each
+// t it
+//   1. loads U(t) on the tile plus a 2-node x/y halo, recovers the
+//      primitives once per loaded node and stages (u,v,w,T), (p,mu) and U in
+//      LDS (two plane slots, ping-pong);
+//   2. computes the x and y convective fluxes of plane t;
+//   3. computes the point fluxes fD, gD (tile + 1-node halo) and hD (tile)
+//      of plane t-1, then the x and y divergences of plane t-1;
+//   4. completes plane t-2 with its z convective flux and z divergence
+//      (register windows along the column), adds the source and writes the
+//      RK update.
+// HBM traffic per node and stage: U in (40 B), U out (40 B), plus the base
+// state in stage 2 (40 B) and the x/y halo re-reads, which hit L2.
+#include "cc_device.h"
+#include "cc_kernels.h"
+
+namespace cc {
+
+template <typename T>
+struct FArgs {
+  Geom g;
+  Phys ph;
+  const double* __restrict__ Uin;
+  const double* Ubase;
+  double* Uout;
+  const double* __restrict__ pw;  // wall pressure: bottom [plane], top [plane]
+  const double* __restrict__ jac;
+  const double* __restrict__ invJac;
+  const double* __restrict__ force;
+  int stage, kchunk, ntx, nty, nz;
+  const int* halt;
+  unsigned long long* blow;
+  int step, n1, n2;
+};
+
+template <typename T>
+__device__ __forceinline__ T quickT(T ue, T uw, T qmm, T qm, T qc, T qp, T qpp) {
+  // face-upwinded QUICK pair (main_v1.9.f90:339-361): the east face takes
+  // its upwind-biased value from the side ue blows from, the west face from
+  // the side uw blows from.
+  const T fe = (ue >= T(0)) ? (T(6) * qc + T(3) * qp - qm) : (T(6) * qp + T(3) * qc - qpp);
+  const T fw = (uw >= T(0)) ? (T(6) * qm + T(3) * qc - qmm) : (T(6) * qc + T(3) * qm - qp);
+  return ue * fe - uw * fw;
+}
+
+template <typename T>
+struct Deriv {
+  T dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, dTdx, dTdy, dTdz;
+};
+
+template <typename T>
+__device__ __forceinline__ T pow07(T x);
+template <>
+__device__ __forceinline__ double pow07<double>(double x) {
+  return pow(x, 0.7);
+}
+template <>
+__device__ __forceinline__ float pow07<float>(float x) {
+  return __expf(0.7f * __logf(x));
+}
+
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
+  constexpr int NT = TX * TY;
+  constexpr int RX = TX + 4, RY = TY + 4, NR = RX * RY;
+  constexpr int NRING = NR - NT;                // halo-2 ring points
+  constexpr int NFX = (TX + 2) * TY;            // fD points (tile + x halo 1)
+  constexpr int NGY = TX * (TY + 2);            // gD points (tile + y halo 1)
+  constexpr int NFG = NFX > NGY ? NFX : NGY;
+  constexpr int NUB = (5 * NR > 4 * NFG) ? 5 * NR : 4 * NFG;
+  constexpr int NZR = 2 * TY + 2 * TX;          // x- and y- ring-1 points
+  static_assert(NRING <= 2 * NT, "tile too small for two ring passes");
+
+  __shared__ T sl[2][4][NR];    // u, v, w, T of two planes
+  __shared__ T spm[2][2][NR];   // p, mu of two planes
+  __shared__ T sub[NUB];        // U(5) of plane t, then fD / gD of plane t-1
+  __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
+
+  const Geom g = a.g;
+  const Phys& ph = a.ph;
+  {
+    const int h = *a.halt;
+    if (h != 0 && h < a.step) return;
+  }
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  int bid = blockIdx.x;
+  const int tile_x = bid % a.ntx;
+  bid /= a.ntx;
+  const int tile_y = bid % a.nty;
+  const int zc = bid / a.nty;
+  const int i0 = tile_x * TX, j0 = tile_y * TY;
+  const int kb = 1 + zc * a.kchunk;
+  const int ke = min(kb + a.kchunk, g.n3 - 1);
+  if (kb >= ke) return;
+  const int P1 = g.P1, P2 = g.P2, n3 = g.n3;
+  const long long F = g.field, PL = g.plane;
+  const bool own_valid = (i0 + tx < P1) && (j0 + ty < P2);
+
+  auto wrapP = [](int x, int P) {
+    x %= P;
+    return x < 0 ? x + P : x;
+  };
+  // region index of the thread's own node and its up to two ring nodes
+  const int r_own = (ty + 2) * RX + (tx + 2);
+  const int gi_own = wrapP(i0 + tx, P1), gj_own = wrapP(j0 + ty, P2);
+  const long long o_own = (long long)gj_own * P1 + gi_own;
+  int r_ring[2];
+  long long o_ring[2];
+  int nring = 0;
+  for (int q = tid; q < NRING && nring < 2; q += NT) {
+    int rx, ry;
+    if (q < 2 * RX) {  // rows 0,1
+      ry = q / RX;
+      rx = q % RX;
+    } else if (q < 4 * RX) {  // rows TY+2, TY+3
+      const int m = q - 2 * RX;
+      ry = TY + 2 + m / RX;
+      rx = m % RX;
+    } else {  // middle rows, columns 0,1,TX+2,TX+3
+      const int m = q - 4 * RX;
+      ry = 2 + m / 4;
+      const int c = m % 4;
+      rx = c < 2 ? c : TX + c;
+    }
+    r_ring[nring] = ry * RX + rx;
+    o_ring[nring] = (long long)wrapP(j0 + ry - 2, P2) * P1 + wrapP(i0 + rx - 2, P1);
+    nring++;
+  }
+  // ring-1 slot of a region index (x ring: rx in {1, TX+2}; y ring: ry in {1, TY+2})
+  auto zring = [&](int rx, int ry) -> int {
+    if ((rx == 1 || rx == TX + 2) && ry >= 2 && ry < TY + 2) return (rx == 1 ? 0 : TY) + (ry - 2);
+    if ((ry == 1 || ry == TY + 2) && rx >= 2 && rx < TX + 2)
+      return 2 * TY + (ry == 1 ? 0 : TX) + (rx - 2);
+    return -1;
+  };
+  // register windows along the own column
+  double Uw[5][5];  // U at planes t-4..t (index 4 = t)
+  T ww[5];          // w at t-4..t
+  T muw[3], uw3[3]; // mu, u at t-2..t
+  T zU[4];          // u,v,w,T at plane t-2
+  T zB[4];          // u,v,w,T at plane t-1
+  T zC[4];          // u,v,w,T at plane t
+  T hDw[3][4];      // hD at t-3, t-2, t-1 (index 2 = newest)
+  T Sg[3][5];       // accumulated flux sums of planes t-2, t-1, t
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    ww[p] = T(0);
+#pragma unroll
+    for (int c = 0; c < 5; c++) Uw[p][c] = 0.0;
+  }
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    muw[p] = T(0);
+    uw3[p] = T(0);
+#pragma unroll
+    for (int c = 0; c < 4; c++) hDw[p][c] = T(0);
+#pragma unroll
+    for (int c = 0; c < 5; c++) Sg[p][c] = T(0);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    zU[c] = T(0);
+    zB[c] = T(0);
+    zC[c] = T(0);
+  }
+
+  const T invRe = T(ph.invRe), ggM = T(ph.ggM), half = T(0.5);
+  const T t23 = T(kTwo3rd), gkRP = T(kGama * ph.invRe * kInvPr);
+  const T invdx = T(ph.invdx), invdy = T(ph.invdy), invdXi = T(ph.invdXi);
+  const T hx = half * invdx, hy = half * invdy;
+
+  // primitives of a loaded node; walls take the boundary values
+  // (main_v1.9.f90:624-643)
+  auto prim = [&](const double* U, long long off, int t, T& u, T& v, T& w, T& Tt, T& p, T& mu,
+                  double (&Uo)[5]) {
+    const long long n = (long long)t * PL + off;
+#pragma unroll
+    for (int c = 0; c < 5; c++) Uo[c] = U[c * F + n];
+    if (t == 0 || t == n3 - 1) {
+      u = v = w = T(0);
+      Tt = T(1);
+      mu = T(1);
+      p = T(a.pw[(t == 0 ? 0 : PL) + off]);
+    } else {
+      const T r = T(Uo[0]);
+      const T ir = T(1) / r;
+      u = T(Uo[1]) * ir;
+      v = T(Uo[2]) * ir;
+      w = T(Uo[3]) * ir;
+      const T E = T(Uo[4]) * ir;
+      Tt = E - ggM * half * (u * u + v * v + w * w);
+      p = r * Tt / T(ph.gM2);
+      mu = pow07<T>(Tt);
+    }
+  };
+
+  const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
+  for (int t = tb; t <= tend; t++) {
+    const int s = t & 1;
+    const bool have_plane = t <= n3 - 1;
+    // ---- phase 1: load plane t ----------------------------------------
+    if (have_plane) {
+      T u, v, w, Tt, p, mu;
+      double Uo[5];
+      prim(a.Uin, o_own, t, u, v, w, Tt, p, mu, Uo);
+      sl[s][0][r_own] = u;
+      sl[s][1][r_own] = v;
+      sl[s][2][r_own] = w;
+      sl[s][3][r_own] = Tt;
+      spm[s][0][r_own] = p;
+      spm[s][1][r_own] = mu;
+#pragma unroll
+      for (int c = 0; c < 5; c++) sub[c * NR + r_own] = T(Uo[c]);
+      // shift the column windows
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        ww[q] = ww[q + 1];
+#pragma unroll
+        for (int c = 0; c < 5; c++) Uw[q][c] = Uw[q + 1][c];
+      }
+      ww[4] = w;
+#pragma unroll
+      for (int c = 0; c < 5; c++) Uw[4][c] = Uo[c];
+      muw[0] = muw[1];
+      muw[1] = muw[2];
+      muw[2] = mu;
+      uw3[0] = uw3[1];
+      uw3[1] = uw3[2];
+      uw3[2] = u;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        zU[c] = zB[c];
+        zB[c] = zC[c];
+      }
+      zC[0] = u;
+      zC[1] = v;
+      zC[2] = w;
+      zC[3] = Tt;
+      for (int q = 0; q < nring; q++) {
+        prim(a.Uin, o_ring[q], t, u, v, w, Tt, p, mu, Uo);
+        const int r = r_ring[q];
+        sl[s][0][r] = u;
+        sl[s][1][r] = v;
+        sl[s][2][r] = w;
+        sl[s][3][r] = Tt;
+        spm[s][0][r] = p;
+        spm[s][1][r] = mu;
+#pragma unroll
+        for (int c = 0; c < 5; c++) sub[c * NR + r] = T(Uo[c]);
+        const int zr = zring(r % RX, r / RX);
+        if (zr >= 0) {
+          srz[t % 3][0][zr] = u;
+          srz[t % 3][1][zr] = v;
+          srz[t % 3][2][zr] = w;
+        }
+      }
+    } else {
+      // past the top wall: shift the windows by one plane of zeros
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        ww[q] = ww[q + 1];
+#pragma unroll
+        for (int c = 0; c < 5; c++) Uw[q][c] = Uw[q + 1][c];
+      }
+      muw[0] = muw[1];
+      muw[1] = muw[2];
+      uw3[0] = uw3[1];
+      uw3[1] = uw3[2];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        zU[c] = zB[c];
+        zB[c] = zC[c];
+      }
+    }
+    // shift the flux-sum window: Sg[2] becomes plane t
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      Sg[0][c] = Sg[1][c];
+      Sg[1][c] = Sg[2][c];
+      Sg[2][c] = T(0);
+    }
+    __syncthreads();
+
+    // ---- phase 2: x, y convective fluxes of plane t (main_v1.9.f90:333-407)
+    if (have_plane && t >= kb && t < ke) {
+      const T* U5 = sub;
+      const int ro = r_own;
+      const T mu = muw[2];
+      // x
+      {
+        const T ru = U5[1 * NR + ro];
+        const T* ul = sl[s][0];
+        if (T(ph.Re) * fabs(ru) * T(ph.dx) > T(ph.Pex) * mu) {
+          const T t1 = (ul[ro] + ul[ro + 1]) * half, t2 = (ul[ro] + ul[ro - 1]) * half;
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const T* q = U5 + c * NR + ro;
+            Sg[2][c] += quickT<T>(t1, t2, q[-2], q[-1], q[0], q[1], q[2]) * (invdx * T(0.125));
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const T* q = U5 + c * NR + ro;
+            Sg[2][c] += (-ul[ro + 2] * q[2] + T(8) * ul[ro + 1] * q[1] - T(8) * ul[ro - 1] * q[-1] +
+                         ul[ro - 2] * q[-2]) * (invdx * T(kOne12th));
+          }
+        }
+      }
+      // y
+      {
+        const T rv = U5[2 * NR + ro];
+        const T* vl = sl[s][1];
+        if (T(ph.Re) * fabs(rv) * T(ph.dy) > T(ph.Pey) * mu) {
+          const T t1 = (vl[ro] + vl[ro + RX]) * half, t2 = (vl[ro] + vl[ro - RX]) * half;
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const T* q = U5 + c * NR + ro;
+            Sg[2][c] += quickT<T>(t1, t2, q[-2 * RX], q[-RX], q[0], q[RX], q[2 * RX]) *
+                        (invdy * T(0.125));
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const T* q = U5 + c * NR + ro;
+            Sg[2][c] += (-vl[ro + 2 * RX] * q[2 * RX] + T(8) * vl[ro + RX] * q[RX] -
+                         T(8) * vl[ro - RX] * q[-RX] + vl[ro - 2 * RX] * q[-2 * RX]) *
+                        (invdy * T(kOne12th));
+          }
+        }
+      }
+    }
+    __syncthreads();  // sub is reused for fD / gD below
+
+    // ---- phase 3: point fluxes of plane p = t-1 (main_v1.9.f90:457-563)
+    const int p = t - 1;
+    const int sp = p & 1;
+    const bool need_h = p >= kb - 1 && p <= ke && p >= 0;
+    const bool need_fg = p >= kb && p < ke;
+    // z derivative of a ring-1 or own node at plane p
+    auto zder = [&](T qm, T q0, T qp) -> T {
+      if (p == 0) return (qp - q0) / T(ph.dz1);
+      if (p == n3 - 1) return (q0 - qm) / T(ph.dzN);
+      return (qp - qm) * (half * invdXi * T(a.invJac[p]));
+    };
+    // derivatives at a region node of plane p; z values supplied
+    auto derivs = [&](int r, const T zm[3], const T zp[3], T zTm, T zTp, Deriv<T>& D) {
+      const T* u = sl[sp][0];
+      const T* v = sl[sp][1];
+      const T* w = sl[sp][2];
+      const T* Tt = sl[sp][3];
+      D.dudx = (u[r + 1] - u[r - 1]) * hx;
+      D.dvdx = (v[r + 1] - v[r - 1]) * hx;
+      D.dwdx = (w[r + 1] - w[r - 1]) * hx;
+      D.dTdx = (Tt[r + 1] - Tt[r - 1]) * hx;
+      D.dudy = (u[r + RX] - u[r - RX]) * hy;
+      D.dvdy = (v[r + RX] - v[r - RX]) * hy;
+      D.dwdy = (w[r + RX] - w[r - RX]) * hy;
+      D.dTdy = (Tt[r + RX] - Tt[r - RX]) * hy;
+      D.dudz = zder(zm[0], u[r], zp[0]);
+      D.dvdz = zder(zm[1], v[r], zp[1]);
+      D.dwdz = zder(zm[2], w[r], zp[2]);
+      D.dTdz = zder(zTm, Tt[r], zTp);
+    };
+    T Fx[4] = {T(0), T(0), T(0), T(0)}, Gy[4] = {T(0), T(0), T(0), T(0)};
+    if (need_h) {
+      // own node: all three point fluxes; hD stays in registers
+      const int r = r_own;
+      const T zm[3] = {zU[0], zU[1], zU[2]}, zp[3] = {zC[0], zC[1], zC[2]};
+      Deriv<T> D;
+      derivs(r, zm, zp, zU[3], zC[3], D);
+      const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
+      const T pr = spm[sp][0][r], mu = spm[sp][1][r];
+      const T divg = D.dudx + D.dvdy + D.dwdz;
+      const T mR = mu * invRe;
+      const T sxz = D.dwdx + D.dudz, syz = D.dwdy + D.dvdz, sxy = D.dvdx + D.dudy;
+      const T dissH = ggM * invRe * mu * (-u * sxz - v * syz + t23 * w * divg - T(2) * w * D.dwdz);
+      T h[4];
+      h[0] = -mR * sxz;
+      h[1] = -mR * syz;
+      h[2] = pr + t23 * mR * divg - T(2) * mR * D.dwdz;
+      h[3] = -gkRP * mu * D.dTdz + ggM * w * pr + dissH;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        hDw[0][c] = hDw[1][c];
+        hDw[1][c] = hDw[2][c];
+        hDw[2][c] = h[c];
+      }
+      if (need_fg) {
+        const T dissF = ggM * invRe * mu * (t23 * u * divg - T(2) * u * D.dudx - v * sxy - w * sxz);
+        const T dissG = ggM * invRe * mu * (-u * sxy + t23 * v * divg - T(2) * v * D.dvdy - w * syz);
+        Fx[0] = pr + t23 * mR * divg - T(2) * mR * D.dudx;
+        Fx[1] = -mR * sxy;
+        Fx[2] = -mR * sxz;
+        Fx[3] = -gkRP * mu * D.dTdx + ggM * u * pr + dissF;
+        Gy[0] = -mR * sxy;
+        Gy[1] = pr + t23 * mR * divg - T(2) * mR * D.dvdy;
+        Gy[2] = -mR * syz;
+        Gy[3] = -gkRP * mu * D.dTdy + ggM * v * pr + dissG;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        hDw[0][c] = hDw[1][c];
+        hDw[1][c] = hDw[2][c];
+        hDw[2][c] = T(0);
+      }
+    }
+    if (need_fg) {
+      // fD on the tile and its x ring-1 columns
+      T* fx = sub;  // [4][(TX+2)*TY], index (ty)*(TX+2) + (rx-1)
+      {
+        const int li = ty * (TX + 2) + (tx + 1);
+#pragma unroll
+        for (int c = 0; c < 4; c++) fx[c * NFX + li] = Fx[c];
+      }
+      if (tid < 2 * TY) {
+        const int side = tid / TY, yy = tid % TY;
+        const int rx = side ? TX + 2 : 1, ry = yy + 2;
+        const int r = ry * RX + rx;
+        const int zr = zring(rx, ry);
+        const int tm = (p - 1 + 3) % 3, tp = (p + 1) % 3;
+        const T zm[3] = {srz[tm][0][zr], srz[tm][1][zr], srz[tm][2][zr]};
+        const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
+        Deriv<T> D;
+        derivs(r, zm, zp, T(0), T(0), D);
+        const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
+        const T pr = spm[sp][0][r], mu = spm[sp][1][r];
+        const T divg = D.dudx + D.dvdy + D.dwdz;
+        const T mR = mu * invRe;
+        const T sxz = D.dwdx + D.dudz, sxy = D.dvdx + D.dudy;
+        const T dissF = ggM * invRe * mu * (t23 * u * divg - T(2) * u * D.dudx - v * sxy - w * sxz);
+        const int li = yy * (TX + 2) + (rx - 1);
+        fx[0 * NFX + li] = pr + t23 * mR * divg - T(2) * mR * D.dudx;
+        fx[1 * NFX + li] = -mR * sxy;
+        fx[2 * NFX + li] = -mR * sxz;
+        fx[3 * NFX + li] = -gkRP * mu * D.dTdx + ggM * u * pr + dissF;
+      }
+    }
+    __syncthreads();
+    if (need_fg) {
+      const T* fx = sub;
+      const int li = ty * (TX + 2) + (tx + 1);
+#pragma unroll
+      for (int c = 0; c < 4; c++) Sg[1][c + 1] += (fx[c * NFX + li + 1] - fx[c * NFX + li - 1]) * hx;
+    }
+    __syncthreads();
+    if (need_fg) {
+      // gD on the tile and its y ring-1 rows
+      T* gy = sub;  // [4][TX*(TY+2)], index (ry-1)*TX + tx
+      {
+        const int li = (ty + 1) * TX + tx;
+#pragma unroll
+        for (int c = 0; c < 4; c++) gy[c * NGY + li] = Gy[c];
+      }
+      if (tid < 2 * TX) {
+        const int side = tid / TX, xx = tid % TX;
+        const int ry = side ? TY + 2 : 1, rx = xx + 2;
+        const int r = ry * RX + rx;
+        const int zr = zring(rx, ry);
+        const int tm = (p - 1 + 3) % 3, tp = (p + 1) % 3;
+        const T zm[3] = {srz[tm][0][zr], srz[tm][1][zr], srz[tm][2][zr]};
+        const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
+        Deriv<T> D;
+        derivs(r, zm, zp, T(0), T(0), D);
+        const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
+        const T pr = spm[sp][0][r], mu = spm[sp][1][r];
+        const T divg = D.dudx + D.dvdy + D.dwdz;
+        const T mR = mu * invRe;
+        const T syz = D.dwdy + D.dvdz, sxy = D.dvdx + D.dudy;
+        const T dissG = ggM * invRe * mu * (-u * sxy + t23 * v * divg - T(2) * v * D.dvdy - w * syz);
+        const int li = (ry - 1) * TX + xx;
+        gy[0 * NGY + li] = -mR * sxy;
+        gy[1 * NGY + li] = pr + t23 * mR * divg - T(2) * mR * D.dvdy;
+        gy[2 * NGY + li] = -mR * syz;
+        gy[3 * NGY + li] = -gkRP * mu * D.dTdy + ggM * v * pr + dissG;
+      }
+    }
+    __syncthreads();
+    if (need_fg) {
+      const T* gy = sub;
+      const int li = (ty + 1) * TX + tx;
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        Sg[1][c + 1] += (gy[c * NGY + li + TX] - gy[c * NGY + li - TX]) * hy;
+    }
+
+    // ---- phase 4: complete plane q = t-2 ------------------------------
+    const int q = t - 2;
+    if (q >= kb && q < ke) {
+      const T iJ = T(a.invJac[q]);
+      // z convective (main_v1.9.f90:409-450); column values at q-2..q+2 are
+      // window slots 0..4 (slot 2 = q)
+      T H[5];
+      const bool lowhigh = (q + 1 < 3) || (q + 1 > n3 - 2);
+      const T Pz = T(ph.Re) * fabs(T(Uw[2][3])) * T(ph.dXi) * T(a.jac[q]);
+      if (lowhigh) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) H[c] = (ww[3] * T(Uw[3][c]) - ww[1] * T(Uw[1][c])) * (invdXi * iJ * half);
+      } else if (Pz > T(ph.Pez) * muw[0]) {
+        const T t1 = (ww[2] + ww[3]) * half, t2 = (ww[2] + ww[1]) * half;
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          H[c] = quickT<T>(t1, t2, T(Uw[0][c]), T(Uw[1][c]), T(Uw[2][c]), T(Uw[3][c]), T(Uw[4][c])) *
+                 (invdXi * iJ * T(0.125));
+      } else {
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          H[c] = (-ww[4] * T(Uw[4][c]) + T(8) * ww[3] * T(Uw[3][c]) - T(8) * ww[1] * T(Uw[1][c]) +
+                  ww[0] * T(Uw[0][c])) * (invdXi * iJ * T(kOne12th));
+      }
+      // z divergence of hD: hD(q+1) - hD(q-1) = slots 2 and 0
+      double R[5];
+      const double tdiff = a.force[0] - a.force[1];
+      const double rb = a.force[2 + q];
+      const double src1 = 0.5 * tdiff * rb;
+      const double src4 = ph.ggM * (double)uw3[0] * 0.5 * tdiff * rb;
+      R[0] = -(double)(Sg[0][0] + H[0]);
+#pragma unroll
+      for (int c = 1; c < 5; c++)
+        R[c] = -(double)(Sg[0][c] + H[c] + (hDw[2][c - 1] - hDw[0][c - 1]) * (half * invdXi * iJ));
+      R[1] += src1;
+      R[4] += src4;
+      if (own_valid) {
+        const long long n = (long long)q * PL + o_own;
+        double Un[5];
+        if (a.stage == 1) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Un[c] = Uw[2][c] + 0.5 * ph.dt * R[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Un[c] = a.Ubase[c * F + n] + ph.dt * R[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 5; c++) a.Uout[c * F + n] = Un[c];
+        if (a.stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
+          const double ir = 1.0 / Un[0];
+          const double uu = Un[1] * ir, vv = Un[2] * ir, ws = Un[3] * ir;
+          const double Tn = Un[4] * ir - ph.ggM * 0.5 * (uu * uu + vv * vv + ws * ws);
+          if (Un[0] < 0.0 || Tn < 0.0) {
+            const unsigned long long lin =
+                ((unsigned long long)q * a.n2 + (unsigned long long)(gj_own + 1)) * a.n1 + (gi_own + 1);
+            atomicMin(a.blow, lin);
+            atomicCAS((int*)a.halt, 0, a.step);
+          }
+        }
+      }
+    }
+    // the next plane overwrites sl/spm/srz slots and sub read above
+    __syncthreads();
+  }
+}
+
+// Wall values and wall-pressure extrapolation in FP64 for the fused path
+// (main_v1.9.f90:622-716): rw <- gama Ma^2 p_old / T, then the new wall
+// pressure from the z-momentum balance at an impervious wall.
+__global__ void __launch_bounds__(256) k_wall_fast(Geom g, Phys ph, const double* __restrict__ U,
+                                                   double* __restrict__ pw, double* __restrict__ rw,
+                                                   int check, int* halt, unsigned long long* blow,
+                                                   int step, int n1, int n2) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  const int wall = blockIdx.z;  // 0 bottom, 1 top
+  if (ii >= g.P1 || jj >= g.P2) return;
+  {
+    const int h = *halt;
+    if (h != 0 && h < step) return;
+  }
+  const long long F = g.field, PL = g.plane;
+  const int N = g.n3 - 1;
+  const long long n2d = (long long)jj * g.P1 + ii;
+  const double pold = pw[wall * PL + n2d];
+  const double r = ph.gM2 * pold;
+  rw[wall * PL + n2d] = r;
+  if (check && r < 0.0) {
+    const unsigned long long lin =
+        ((unsigned long long)(wall ? N : 0) * n2 + (unsigned long long)(jj + 1)) * n1 + (ii + 1);
+    atomicMin(blow, lin);
+    atomicCAS(halt, 0, step);
+  }
+  const int k1 = wall ? N - 1 : 1, k2 = wall ? N - 2 : 2;
+  const double c0 = wall ? ph.b1d2c0 : ph.f1d2c0, c1 = wall ? ph.b1d2c1 : ph.f1d2c1,
+               c2 = wall ? ph.b1d2c2 : ph.f1d2c2;
+  const double e1 = wall ? ph.b2d1c1 : ph.f2d1c1, e2 = wall ? ph.b2d1c2 : ph.f2d1c2;
+  struct P {
+    double r, u, v, w, p, mu;
+  };
+  auto prim = [&](int i, int j, int k) {
+    const long long n = (long long)k * PL + (long long)j * g.P1 + i;
+    P q;
+    q.r = U[n];
+    const double ir = 1.0 / q.r;
+    q.u = U[F + n] * ir;
+    q.v = U[2 * F + n] * ir;
+    q.w = U[3 * F + n] * ir;
+    const double T = U[4 * F + n] * ir - ph.ggM * 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w);
+    q.p = q.r * T / ph.gM2;
+    q.mu = pow(T, 0.7);
+    return q;
+  };
+  const int im = wrapi(ii - 1, g.P1), ip = wrapi(ii + 1, g.P1);
+  const int jm = wrapi(jj - 1, g.P2), jp = wrapi(jj + 1, g.P2);
+  // wall node values: u = v = w = 0, mu = 1 -> the k=1 (or n3) terms vanish
+  const P xp1 = prim(ip, jj, k1), xp2 = prim(ip, jj, k2), xm1 = prim(im, jj, k1), xm2 = prim(im, jj, k2);
+  const P yp1 = prim(ii, jp, k1), yp2 = prim(ii, jp, k2), ym1 = prim(ii, jm, k1), ym2 = prim(ii, jm, k2);
+  const P c1p = prim(ii, jj, k1), c2p = prim(ii, jj, k2);
+  const double dudxdz =
+      ((c1 * xp1.mu * xp1.u + c2 * xp2.mu * xp2.u) - (c1 * xm1.mu * xm1.u + c2 * xm2.mu * xm2.u)) *
+      0.5 * ph.invdx;
+  const double dvdydz =
+      ((c1 * yp1.mu * yp1.v + c2 * yp2.mu * yp2.v) - (c1 * ym1.mu * ym1.v + c2 * ym2.mu * ym2.v)) *
+      0.5 * ph.invdy;
+  const double d2wdz2 = e1 * c1p.mu * c1p.w + e2 * c2p.mu * c2p.w;
+  const double drhow2dz = c1 * c1p.r * c1p.w * c1p.w + c2 * c2p.r * c2p.w * c2p.w;
+  const double lhs =
+      ph.invRe * (-dudxdz - dvdydz + kTwo3rd * (dudxdz + dvdydz - 2.0 * d2wdz2)) + drhow2dz;
+  pw[wall * PL + n2d] = -(lhs + c1 * c1p.p + c2 * c2p.p) / c0;
+}
+
+template <typename T, int TX, int TY>
+static void launch_t(const Geom& g, const Phys& ph, const FusedCfg& cfg, const double* Uin,
+                     const double* Ubase, double* Uout, const double* pw, const double* jac,
+                     const double* invJac, const double* force, int stage, int* halt,
+                     unsigned long long* blow, int step, int n1, int n2, hipStream_t s) {
+  FArgs<T> a;
+  a.g = g;
+  a.ph = ph;
+  a.Uin = Uin;
+  a.Ubase = Ubase;
+  a.Uout = Uout;
+  a.pw = pw;
+  a.jac = jac;
+  a.invJac = invJac;
+  a.force = force;
+  a.stage = stage;
+  a.ntx = (g.P1 + TX - 1) / TX;
+  a.nty = (g.P2 + TY - 1) / TY;
+  const int nint = g.n3 - 2;
+  int kc = cfg.kchunk > 0 ? cfg.kchunk : 0;
+  if (kc == 0) {  // enough blocks to cover the chip at one block per CU
+    const int tiles = a.ntx * a.nty;
+    const int nzw = (512 + tiles - 1) / tiles;
+    kc = (nint + nzw - 1) / nzw;
+    if (kc < 8) kc = 8;
+  }
+  a.kchunk = kc;
+  a.nz = (nint + kc - 1) / kc;
+  a.halt = halt;
+  a.blow = blow;
+  a.step = step;
+  a.n1 = n1;
+  a.n2 = n2;
+  const unsigned nblk = (unsigned)(a.ntx * a.nty * a.nz);
+  hipLaunchKernelGGL((k_stage_fused<T, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, a);
+}
+
+void launch_stage_fused(const Geom& g, const Phys& ph, int mixed, const FusedCfg& cfg,
+                        const double* Uin, const double* Ubase, double* Uout, double* pw,
+                        double* rw, const double* jac, const double* invJac, const double* force,
+                        int stage, int* halt, unsigned long long* blow, int step, int n1, int n2,
+                        hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+  if (ev0) (void)hipEventRecord(ev0, s);
+  if (mixed)
+    launch_t<float, 32, 16>(g, ph, cfg, Uin, Ubase, Uout, pw, jac, invJac, force, stage, halt,
+                            blow, step, n1, n2, s);
+  else
+    launch_t<double, 32, 16>(g, ph, cfg, Uin, Ubase, Uout, pw, jac, invJac, force, stage, halt,
+                             blow, step, n1, n2, s);
+  if (ev1) (void)hipEventRecord(ev1, s);
+  dim3 blk(64, 4);
+  dim3 grd((g.P1 + 63) / 64, (g.P2 + 3) / 4, 2);
+  hipLaunchKernelGGL(k_wall_fast, grd, blk, 0, s, g, ph, Uout, pw, rw, stage == 2, halt, blow,
+                     step, n1, n2);
+}
+
+}  // namespace cc
