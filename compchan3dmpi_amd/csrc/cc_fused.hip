@@ -60,15 +60,64 @@ struct Deriv {
   T dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, dTdx, dTdy, dTdz;
 };
 
+// T**0.7 for T > 0 (mu = kt = T**0.7, main_v1.9.f90:643,720).
+// Double: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+// s = (m-1)/(m+1), |s| < 0.172, by its odd series to s^21; then
+// 2^(0.7 log2 x) = 2^n exp(f ln 2) with n = rint(.), |f| <= 1/2 and the
+// exponential by its Taylor series to degree 14.  Error <= 2 ulp over the
+// range a temperature can take, which the 1e-12 FP64 tolerance absorbs.
+// Float: the hardware log2/exp2.
+__device__ __forceinline__ double pow07d(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double lnm = fma(2.0 * s * s2, p, 2.0 * s);
+  constexpr double kLn2 = 0.69314718055994530942;
+  const double y = fma(0.7 / kLn2, lnm, 0.7 * (double)e);
+  const double n = rint(y);
+  const double g = (y - n) * kLn2;
+  double q = 1.0 / 87178291200.0;  // 1/14!
+  q = fma(q, g, 1.0 / 6227020800.0);
+  q = fma(q, g, 1.0 / 479001600.0);
+  q = fma(q, g, 1.0 / 39916800.0);
+  q = fma(q, g, 1.0 / 3628800.0);
+  q = fma(q, g, 1.0 / 362880.0);
+  q = fma(q, g, 1.0 / 40320.0);
+  q = fma(q, g, 1.0 / 5040.0);
+  q = fma(q, g, 1.0 / 720.0);
+  q = fma(q, g, 1.0 / 120.0);
+  q = fma(q, g, 1.0 / 24.0);
+  q = fma(q, g, 1.0 / 6.0);
+  q = fma(q, g, 0.5);
+  q = fma(q, g, 1.0);
+  q = fma(q, g, 1.0);
+  return ldexp(q, (int)n);
+}
+
 template <typename T>
 __device__ __forceinline__ T pow07(T x);
 template <>
 __device__ __forceinline__ double pow07<double>(double x) {
-  return pow(x, 0.7);
+  return pow07d(x);
 }
 template <>
 __device__ __forceinline__ float pow07<float>(float x) {
-  return __expf(0.7f * __logf(x));
+  return exp2f(0.7f * log2f(x));
 }
 
 template <typename T, int TX, int TY>
@@ -79,14 +128,15 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
   constexpr int NFX = (TX + 2) * TY;            // fD points (tile + x halo 1)
   constexpr int NGY = TX * (TY + 2);            // gD points (tile + y halo 1)
   constexpr int NFG = NFX > NGY ? NFX : NGY;
-  constexpr int NUB = (5 * NR > 4 * NFG) ? 5 * NR : 4 * NFG;
+  constexpr int NUB = (5 * NR > 4 * (NFX + NGY)) ? 5 * NR : 4 * (NFX + NGY);
   constexpr int NZR = 2 * TY + 2 * TX;          // x- and y- ring-1 points
   static_assert(NRING <= 2 * NT, "tile too small for two ring passes");
 
   __shared__ T sl[2][4][NR];    // u, v, w, T of two planes
   __shared__ T spm[2][2][NR];   // p, mu of two planes
-  __shared__ T sub[NUB];        // U(5) of plane t, then fD / gD of plane t-1
+  __shared__ T sub[NUB];        // U(5) of plane t, then fD and gD of plane t-1
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
+  __shared__ T shd[3][4][NT];   // hD of the own column, planes mod 3
 
   const Geom g = a.g;
   const Phys& ph = a.ph;
@@ -96,7 +146,12 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
   }
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
+  // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each
+  // XCD walks a contiguous range of tiles and their shared halos stay in
+  // that XCD's L2.
+  const int nblk = a.ntx * a.nty * a.nz;
   int bid = blockIdx.x;
+  if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
   const int tile_x = bid % a.ntx;
   bid /= a.ntx;
   const int tile_y = bid % a.nty;
@@ -149,11 +204,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
   // register windows along the own column
   double Uw[5][5];  // U at planes t-4..t (index 4 = t)
   T ww[5];          // w at t-4..t
-  T muw[3], uw3[3]; // mu, u at t-2..t
-  T zU[4];          // u,v,w,T at plane t-2
-  T zB[4];          // u,v,w,T at plane t-1
-  T zC[4];          // u,v,w,T at plane t
-  T hDw[3][4];      // hD at t-3, t-2, t-1 (index 2 = newest)
+  unsigned zq = 0;  // bit b: plane t-b uses QUICK in z (Pe_z > PezVal)
+  T zU[4];          // u,v,w,T at plane t-2 (t-1 and t are in the LDS slots)
   T Sg[3][5];       // accumulated flux sums of planes t-2, t-1, t
 #pragma unroll
   for (int p = 0; p < 5; p++) {
@@ -163,24 +215,17 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
   }
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    muw[p] = T(0);
-    uw3[p] = T(0);
-#pragma unroll
-    for (int c = 0; c < 4; c++) hDw[p][c] = T(0);
 #pragma unroll
     for (int c = 0; c < 5; c++) Sg[p][c] = T(0);
   }
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    zU[c] = T(0);
-    zB[c] = T(0);
-    zC[c] = T(0);
-  }
+  for (int c = 0; c < 4; c++) zU[c] = T(0);
 
   const T invRe = T(ph.invRe), ggM = T(ph.ggM), half = T(0.5);
   const T t23 = T(kTwo3rd), gkRP = T(kGama * ph.invRe * kInvPr);
   const T invdx = T(ph.invdx), invdy = T(ph.invdy), invdXi = T(ph.invdXi);
   const T hx = half * invdx, hy = half * invdy;
+  const T igM2 = T(1.0 / ph.gM2);
 
   // primitives of a loaded node; walls take the boundary values
   // (main_v1.9.f90:624-643)
@@ -202,7 +247,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
       w = T(Uo[3]) * ir;
       const T E = T(Uo[4]) * ir;
       Tt = E - ggM * half * (u * u + v * v + w * w);
-      p = r * Tt / T(ph.gM2);
+      p = r * Tt * igM2;
       mu = pow07<T>(Tt);
     }
   };
@@ -212,6 +257,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
     const int s = t & 1;
     const bool have_plane = t <= n3 - 1;
     // ---- phase 1: load plane t ----------------------------------------
+    // plane t-2 of the own column leaves the LDS slot this iteration
+#pragma unroll
+    for (int c = 0; c < 4; c++) zU[c] = sl[s][c][r_own];
     if (have_plane) {
       T u, v, w, Tt, p, mu;
       double Uo[5];
@@ -234,21 +282,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
       ww[4] = w;
 #pragma unroll
       for (int c = 0; c < 5; c++) Uw[4][c] = Uo[c];
-      muw[0] = muw[1];
-      muw[1] = muw[2];
-      muw[2] = mu;
-      uw3[0] = uw3[1];
-      uw3[1] = uw3[2];
-      uw3[2] = u;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        zU[c] = zB[c];
-        zB[c] = zC[c];
-      }
-      zC[0] = u;
-      zC[1] = v;
-      zC[2] = w;
-      zC[3] = Tt;
+      // z scheme of this plane (main_v1.9.f90:409-416)
+      const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(a.jac[t]) > T(ph.Pez) * mu;
+      zq = (zq << 1) | (qz ? 1u : 0u);
       for (int q = 0; q < nring; q++) {
         prim(a.Uin, o_ring[q], t, u, v, w, Tt, p, mu, Uo);
         const int r = r_ring[q];
@@ -275,15 +311,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < 5; c++) Uw[q][c] = Uw[q + 1][c];
       }
-      muw[0] = muw[1];
-      muw[1] = muw[2];
-      uw3[0] = uw3[1];
-      uw3[1] = uw3[2];
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        zU[c] = zB[c];
-        zB[c] = zC[c];
-      }
+      zq <<= 1;
     }
     // shift the flux-sum window: Sg[2] becomes plane t
 #pragma unroll
@@ -298,7 +326,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
     if (have_plane && t >= kb && t < ke) {
       const T* U5 = sub;
       const int ro = r_own;
-      const T mu = muw[2];
+      const T mu = spm[s][1][ro];
       // x
       {
         const T ru = U5[1 * NR + ro];
@@ -378,9 +406,10 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
     if (need_h) {
       // own node: all three point fluxes; hD stays in registers
       const int r = r_own;
-      const T zm[3] = {zU[0], zU[1], zU[2]}, zp[3] = {zC[0], zC[1], zC[2]};
+      const T zm[3] = {zU[0], zU[1], zU[2]};
+      const T zp[3] = {sl[s][0][r], sl[s][1][r], sl[s][2][r]};
       Deriv<T> D;
-      derivs(r, zm, zp, zU[3], zC[3], D);
+      derivs(r, zm, zp, zU[3], sl[s][3][r], D);
       const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
       const T pr = spm[sp][0][r], mu = spm[sp][1][r];
       const T divg = D.dudx + D.dvdy + D.dwdz;
@@ -393,11 +422,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
       h[2] = pr + t23 * mR * divg - T(2) * mR * D.dwdz;
       h[3] = -gkRP * mu * D.dTdz + ggM * w * pr + dissH;
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        hDw[0][c] = hDw[1][c];
-        hDw[1][c] = hDw[2][c];
-        hDw[2][c] = h[c];
-      }
+      for (int c = 0; c < 4; c++) shd[p % 3][c][tid] = h[c];
       if (need_fg) {
         const T dissF = ggM * invRe * mu * (t23 * u * divg - T(2) * u * D.dudx - v * sxy - w * sxz);
         const T dissG = ggM * invRe * mu * (-u * sxy + t23 * v * divg - T(2) * v * D.dvdy - w * syz);
@@ -409,13 +434,6 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
         Gy[1] = pr + t23 * mR * divg - T(2) * mR * D.dvdy;
         Gy[2] = -mR * syz;
         Gy[3] = -gkRP * mu * D.dTdy + ggM * v * pr + dissG;
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        hDw[0][c] = hDw[1][c];
-        hDw[1][c] = hDw[2][c];
-        hDw[2][c] = T(0);
       }
     }
     if (need_fg) {
@@ -449,17 +467,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
         fx[3 * NFX + li] = -gkRP * mu * D.dTdx + ggM * u * pr + dissF;
       }
     }
-    __syncthreads();
-    if (need_fg) {
-      const T* fx = sub;
-      const int li = ty * (TX + 2) + (tx + 1);
-#pragma unroll
-      for (int c = 0; c < 4; c++) Sg[1][c + 1] += (fx[c * NFX + li + 1] - fx[c * NFX + li - 1]) * hx;
-    }
-    __syncthreads();
     if (need_fg) {
       // gD on the tile and its y ring-1 rows
-      T* gy = sub;  // [4][TX*(TY+2)], index (ry-1)*TX + tx
+      T* gy = sub + 4 * NFX;  // [4][TX*(TY+2)], index (ry-1)*TX + tx
       {
         const int li = (ty + 1) * TX + tx;
 #pragma unroll
@@ -490,11 +500,14 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
     }
     __syncthreads();
     if (need_fg) {
-      const T* gy = sub;
-      const int li = (ty + 1) * TX + tx;
+      const T* fx = sub;
+      const T* gy = sub + 4 * NFX;
+      const int lf = ty * (TX + 2) + (tx + 1);
+      const int lg = (ty + 1) * TX + tx;
 #pragma unroll
       for (int c = 0; c < 4; c++)
-        Sg[1][c + 1] += (gy[c * NGY + li + TX] - gy[c * NGY + li - TX]) * hy;
+        Sg[1][c + 1] += (fx[c * NFX + lf + 1] - fx[c * NFX + lf - 1]) * hx +
+                        (gy[c * NGY + lg + TX] - gy[c * NGY + lg - TX]) * hy;
     }
 
     // ---- phase 4: complete plane q = t-2 ------------------------------
@@ -505,11 +518,10 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
       // window slots 0..4 (slot 2 = q)
       T H[5];
       const bool lowhigh = (q + 1 < 3) || (q + 1 > n3 - 2);
-      const T Pz = T(ph.Re) * fabs(T(Uw[2][3])) * T(ph.dXi) * T(a.jac[q]);
       if (lowhigh) {
 #pragma unroll
         for (int c = 0; c < 5; c++) H[c] = (ww[3] * T(Uw[3][c]) - ww[1] * T(Uw[1][c])) * (invdXi * iJ * half);
-      } else if (Pz > T(ph.Pez) * muw[0]) {
+      } else if (zq & 4u) {
         const T t1 = (ww[2] + ww[3]) * half, t2 = (ww[2] + ww[1]) * half;
 #pragma unroll
         for (int c = 0; c < 5; c++)
@@ -526,11 +538,13 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
       const double tdiff = a.force[0] - a.force[1];
       const double rb = a.force[2 + q];
       const double src1 = 0.5 * tdiff * rb;
-      const double src4 = ph.ggM * (double)uw3[0] * 0.5 * tdiff * rb;
+      const double src4 = ph.ggM * (double)(T(Uw[2][1]) / T(Uw[2][0])) * 0.5 * tdiff * rb;
       R[0] = -(double)(Sg[0][0] + H[0]);
 #pragma unroll
       for (int c = 1; c < 5; c++)
-        R[c] = -(double)(Sg[0][c] + H[c] + (hDw[2][c - 1] - hDw[0][c - 1]) * (half * invdXi * iJ));
+        R[c] = -(double)(Sg[0][c] + H[c] +
+                         (shd[(q + 1) % 3][c - 1][tid] - shd[(q + 2) % 3][c - 1][tid]) *
+                             (half * invdXi * iJ));
       R[1] += src1;
       R[4] += src4;
       if (own_valid) {
