@@ -42,7 +42,7 @@ struct Chan {
   hipStream_t s = nullptr;
   double *UA = nullptr, *UB = nullptr, *W = nullptr, *rw = nullptr, *pw = nullptr;
   double *jac = nullptr, *invJac = nullptr, *force = nullptr, *consts = nullptr;
-  double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr;
+  double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr, *dbuf = nullptr;
   int* halt = nullptr;
   unsigned long long* blow = nullptr;
   int nstep = 0;
@@ -125,7 +125,7 @@ void setup_mesh(Chan& c) {
 void free_chan(Chan* c) {
   if (!c) return;
   double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->jac, &c->invJac, &c->force,
-                     &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red};
+                     &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red, &c->dbuf};
   for (double** b : bufs)
     if (*b) (void)hipFree(*b);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -357,8 +357,24 @@ int cc_profile(void* handle, int mode, double* out) {
 int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
   Chan* c = (Chan*)handle;
   if (!c || !x3_file || !out) return fail(CC_EINVAL, "null argument");
-  (void)x2_end;
-  return fail(CC_EINVAL, "bulk diagnostics are not built into this library");
+  if (!c->loaded) return fail(CC_ESTATE, "no state uploaded");
+  const size_t full = (size_t)c->n1 * c->n2 * c->n3;
+  if (!c->Vdev) HIPCHK(hipMalloc((void**)&c->Vdev, 7 * full * sizeof(double)));
+  if (!c->dbuf) HIPCHK(hipMalloc((void**)&c->dbuf, (1024 + 16 + c->n3) * sizeof(double)));
+  double* part = c->dbuf;
+  double* dout = c->dbuf + 1024;
+  double* x3f = c->dbuf + 1040;
+  HIPCHK(hipMemcpyAsync(x3f, x3_file, c->n3 * sizeof(double), hipMemcpyHostToDevice, c->s));
+  const double* V = c->Vfile;
+  if (!c->fresh) {
+    launch_download(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->Vfile, c->Vdev, c->s);
+    V = c->Vdev;
+  }
+  launch_diag(c->n1, c->n2, c->n3, c->ph.dy, V, part, x3f, x2_end, dout, c->s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, 16 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
 }
 
 void cc_destroy(void* handle) { free_chan((Chan*)handle); }

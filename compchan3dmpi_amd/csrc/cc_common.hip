@@ -278,6 +278,72 @@ __global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int
   }
 }
 
+// ---- bulk diagnostics on the reconstructed var (main_v1.9.f90:799-906) ----
+// Pass 1: per-block partial sums of u^2+v^2+w^2 over every node (halo values
+// included, as the reference's all-node loop).
+__global__ void __launch_bounds__(256) k_ke_partial(long long total, const double* __restrict__ V,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[4];
+  double acc[1] = {0.0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const double u = V[total + t], v = V[2 * total + t], w = V[3 * total + t];
+    acc[0] += u * u + v * v + w * w;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Pass 2 (one block): KE from the partials in a fixed order, then the three
+// bulk plane integrals for each of the five x-planes with the trapezoidal
+// quarter-sum of main_v1.9.f90:827-834.
+__global__ void __launch_bounds__(256) k_diag_final(int n1, int n2, int n3, double dy,
+                                                    const double* __restrict__ V,
+                                                    const double* __restrict__ part, int npart,
+                                                    const double* __restrict__ x3f, double x2e,
+                                                    double* __restrict__ out) {
+  __shared__ double lds[4];
+  const long long total = (long long)n1 * n2 * n3;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < npart; b++) s += part[b];
+    out[0] = s / (double)(n1 * n2 * n3);
+  }
+  const int plane[5] = {5, 10, 70, 100, 125};
+  const double Ayz = x3f[n3 - 1] * x2e;
+  for (int q = 0; q < 3; q++)
+    for (int m = 0; m < 5; m++) {
+      const int p = plane[m];  // 1-based x index
+      double acc[1] = {0.0};
+      if (p > 2 && p < n1 - 1) {
+        const int i = p - 1;
+        const long long nj = n2 - 1, nk = n3 - 1;
+        for (long long t = threadIdx.x; t < nj * nk; t += blockDim.x) {
+          const int j = (int)(t % nj), k = (int)(t / nj);
+          const double dAyz = dy * (x3f[k + 1] - x3f[k]);
+          auto val = [&](int jj, int kk) {
+            const long long n = ((long long)kk * n2 + jj) * n1 + i;
+            if (q == 0) return V[n];
+            if (q == 1) return V[total + n];
+            return V[n] * V[total + n];
+          };
+          acc[0] += 0.25 * (val(j, k) + val(j + 1, k) + val(j, k + 1) + val(j + 1, k + 1)) * dAyz;
+        }
+      }
+      block_sum<1>(acc, lds);
+      if (threadIdx.x == 0) out[1 + 5 * q + m] = acc[0] / Ayz;
+      __syncthreads();
+    }
+}
+
+void launch_diag(int n1, int n2, int n3, double dy, const double* Vdev, double* part,
+                 const double* x3f, double x2e, double* out, hipStream_t s) {
+  const long long total = (long long)n1 * n2 * n3;
+  hipLaunchKernelGGL(k_ke_partial, dim3(1024), dim3(256), 0, s, total, Vdev, part);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, n1, n2, n3, dy, Vdev, part, 1024,
+                     x3f, x2e, out);
+}
+
 void launch_upload(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull, double* UA,
                    double* UB, double* W, WallP pw, double* rw, double* consts, hipStream_t s) {
   const int B = 256;

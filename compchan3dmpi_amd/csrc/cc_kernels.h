@@ -44,6 +44,11 @@ void launch_upload(const Geom& g, const Phys& ph, int n1, int n2, const double* 
 void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double* U,
                      WallP pw, const double* rw, const double* Vfull, double* Vout,
                      hipStream_t s);
+// Bulk diagnostics on the reconstructed var (Vdev, from launch_download):
+// out[16] = KE, rho_b[5], U_b[5], massflow[5]; part holds 1024 partials;
+// x3f (n3) and x2e are the file's mesh values.
+void launch_diag(int n1, int n2, int n3, double dy, const double* Vdev, double* part,
+                 const double* x3f, double x2e, double* out, hipStream_t s);
 // tau[0], tau[1] -> force[0], force[1]; plane statistics -> stats[23][n3]
 // and force[2 + kk] = mean density of plane kk.
 void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,

@@ -1,0 +1,94 @@
+!=======================================================================
+! cc_iface -- ISO_C_BINDING interfaces to the MI355X time step
+! (include/cc_abi.h).  The derived type mirrors cc_params field by field.
+!=======================================================================
+module cc_iface
+use iso_c_binding
+implicit none
+
+integer(c_int), parameter :: CC_FP64_STRICT = 0, CC_FP64 = 1, CC_MIXED = 2
+
+type, bind(C) :: cc_params
+	integer(c_int) :: n1, n2, n3, istat, precision, reserved
+	real(c_double) :: Re, Mac, dt, PexVal, PeyVal, PezVal, Ri_b, slip_len
+end type cc_params
+
+interface
+	integer(c_int) function cc_create(p, h) bind(C, name="cc_create")
+		import :: c_int, c_ptr, cc_params
+		type(cc_params), intent(in) :: p
+		type(c_ptr), intent(out) :: h
+	end function cc_create
+
+	integer(c_int) function cc_upload(h, var, nstep, time) bind(C, name="cc_upload")
+		import :: c_int, c_ptr, c_double
+		type(c_ptr), value :: h
+		real(c_double), intent(in) :: var(*)
+		integer(c_int), value :: nstep
+		real(c_double), value :: time
+	end function cc_upload
+
+	integer(c_int) function cc_step(h, nsteps) bind(C, name="cc_step")
+		import :: c_int, c_ptr
+		type(c_ptr), value :: h
+		integer(c_int), value :: nsteps
+	end function cc_step
+
+	integer(c_int) function cc_stats(h, tau, stat1, stat2, corr) bind(C, name="cc_stats")
+		import :: c_int, c_ptr, c_double
+		type(c_ptr), value :: h
+		real(c_double), intent(out) :: tau(2), stat1(*), stat2(*), corr(*)
+	end function cc_stats
+
+	integer(c_int) function cc_diag(h, x3f, x2e, out) bind(C, name="cc_diag")
+		import :: c_int, c_ptr, c_double
+		type(c_ptr), value :: h
+		real(c_double), intent(in) :: x3f(*)
+		real(c_double), value :: x2e
+		real(c_double), intent(out) :: out(16)
+	end function cc_diag
+
+	integer(c_int) function cc_download(h, var, nstep, time) bind(C, name="cc_download")
+		import :: c_int, c_ptr, c_double
+		type(c_ptr), value :: h
+		real(c_double), intent(out) :: var(*)
+		integer(c_int), intent(out) :: nstep
+		real(c_double), intent(out) :: time
+	end function cc_download
+
+	integer(c_int) function cc_blown(h, ijk) bind(C, name="cc_blown")
+		import :: c_int, c_ptr
+		type(c_ptr), value :: h
+		integer(c_int), intent(out) :: ijk(3)
+	end function cc_blown
+
+	subroutine cc_destroy(h) bind(C, name="cc_destroy")
+		import :: c_ptr
+		type(c_ptr), value :: h
+	end subroutine cc_destroy
+
+	type(c_ptr) function cc_last_error() bind(C, name="cc_last_error")
+		import :: c_ptr
+	end function cc_last_error
+end interface
+
+contains
+
+! message of the last failed call, as a Fortran string
+function cc_errmsg() result(msg)
+	character(len=:), allocatable :: msg
+	type(c_ptr) :: p
+	character(kind=c_char), pointer :: s(:)
+	integer :: n
+	p = cc_last_error()
+	call c_f_pointer(p, s, [512])
+	n = 0
+	do while (n < 512)
+		if (s(n+1) == c_null_char) exit
+		n = n + 1
+	enddo
+	allocate(character(len=n) :: msg)
+	msg = transfer(s(1:n), msg)
+end function cc_errmsg
+
+end module cc_iface

@@ -1,0 +1,342 @@
+!=======================================================================
+! chanCompr -- drop-in driver for the compressible channel DNS whose time
+! step runs on one MI355X through the C ABI of include/cc_abi.h.
+!
+! File contracts are the reference's (theparvezahmad/compChan3Dmpi, v1.9):
+!   in : ../param.dat (comdata.f90:31-68), the solution file named there
+!        (resOp=1, main_v1.9.f90:222-251) or ../output/3D001.dat (resOp=2)
+!   out: ../output/3D001.dat / 3D001_.dat alternating every 1000 steps and
+!        domInfo.txt (main_v1.9.f90:749-794); KE.dat, rhobulk.dat,
+!        Ubulk.dat, massflow.dat every 10 steps (:799-906); tauWall.dat,
+!        moment1st/2nd/Joint.dat every istat steps (:910-1037); the 10-step
+!        timing line on stdout (:1042-1047); on rho<0 or T<0 the offending
+!        nodes on stdout, ../debug/3D001.dat and stop (:709-743).
+! The grid is read from the ZONE line of the solution file instead of the
+! compile-time nP.  Environment (no counterpart in the reference, where
+! these are compile-time): CC_MAXSTEP (default 50000000), CC_PRECISION
+! (fp64 | fp64-strict | mixed, default fp64), CC_NWRITE (default 1000).
+!=======================================================================
+program chanCompr
+use iso_c_binding
+use cc_iface
+implicit none
+
+integer :: resOp, topo(3), istat, debugOp
+double precision :: Re, Mac, dt, PexVal, PeyVal, PezVal
+character(len=10) :: descp_
+character(len=100) :: inputPath_
+character(len=:), allocatable :: descp, inputPath
+
+integer :: n1, n2, n3, i, j, k, st, nstep, nsub, nnext, maxstep, nwrite, rc, prec
+integer(c_int) :: ns_c, ijk(3)
+double precision :: time, t_c, tStart, tEnd, tau(2), dg(16)
+double precision, allocatable :: var(:,:,:,:), x1(:), x2(:), x3(:)
+double precision, allocatable :: stat1(:,:), stat2(:,:), corr(:,:)
+character(len=30) :: filename
+character(len=64) :: env
+character(len=200) :: zone
+type(cc_params) :: prm
+type(c_ptr) :: h
+integer(8) :: clk0, clk1, crate
+logical :: isBlown
+
+!------------------------------- param.dat -------------------------------
+open(unit=10, file="../param.dat", status="old")
+read(10,*)
+read(10,*)
+read(10,'(48X,A)') descp_
+read(10,'(48X,i1)') resOp
+read(10,*)
+read(10,'(48X,i2)') topo(1)
+read(10,'(48X,i2)') topo(2)
+read(10,'(48X,i2)') topo(3)
+read(10,*)
+read(10,*)
+read(10,'(48X,F4.0)') Re
+read(10,'(48X,f3.1)') Mac
+read(10,'(48X,f6.4)') dt
+read(10,'(48X,i2)') istat
+read(10,*)
+read(10,'(48X,f3.1)') PexVal
+read(10,'(48X,f3.1)') PeyVal
+read(10,'(48X,f3.1)') PezVal
+read(10,*)
+read(10,'(48X,i1)') debugOp
+read(10,'(48X,A)') inputPath_
+close(10)
+allocate(character(len=len(trim(descp_))) :: descp)
+allocate(character(len=len(trim(inputPath_))) :: inputPath)
+descp = trim(descp_)
+inputPath = trim(inputPath_)
+
+maxstep = 50000000
+call get_environment_variable("CC_MAXSTEP", env, status=st)
+if (st == 0) read(env,*) maxstep
+nwrite = 1000
+call get_environment_variable("CC_NWRITE", env, status=st)
+if (st == 0) read(env,*) nwrite
+prec = CC_FP64
+call get_environment_variable("CC_PRECISION", env, status=st)
+if (st == 0) then
+	select case (trim(env))
+	case ("fp64-strict")
+		prec = CC_FP64_STRICT
+	case ("mixed")
+		prec = CC_MIXED
+	case default
+		prec = CC_FP64
+	end select
+endif
+
+write(6,'(A)')"==================================================================================="
+write(6,'(2A)')"Compressible Channel Flow Solver started at :",stamp()
+write(6,'(A)')"==================================================================================="
+write(6,'(A)')
+write(6,'(A)')"==================================================================================="
+write(6,'(A)')"Parameters read from param.dat"
+write(6,'(A)')"-----------------------------------------------------------------------------------"
+write(6,'(2A)')"Case Description        : ",descp
+write(6,'(2A)')"Path to input file      : ",inputPath
+write(6,'(A,3(I3,1X))')"Process Topology        : ",topo(1),topo(2),topo(3)
+write(6,'(A,F10.4)'   )"Reynolds Number         : ",Re
+write(6,'(A,F10.2)'   )"Mach Number             : ",Mac
+write(6,'(A,F10.8)'   )"Timestep                : ",dt
+write(6,'(A,I10)'     )"Freq of stat calculation: ",istat
+write(6,'(A,F10.4)'   )"Peclet in x-direction   : ",PexVal
+write(6,'(A,F10.4)'   )"Peclet in y-direction   : ",PeyVal
+write(6,'(A,F10.4)'   )"Peclet in z-direction   : ",PezVal
+write(6,'(A,I10)'     )"Restart Option  [1/2]   : ",resOp
+write(6,'(A,I10)'     )"Debug Option    [0/1]   : ",debugOp
+write(6,'(A)')"==================================================================================="
+
+!----------------------------- solution file -----------------------------
+if (resOp == 2) then
+	filename = "../output/3D001.dat"
+	open(unit=10, file=trim(filename), status="old")
+else
+	open(unit=10, file=inputPath, status="old")
+endif
+read(10,'(8X,F20.10,I9,1X)') time, nstep
+read(10,*)
+read(10,'(A)') zone
+read(zone,'(7X,I3,3X,I3,3X,I3)', iostat=st) n3, n2, n1
+if (st /= 0) then
+	write(6,'(A)') "Error in reading the ZONE line of the input file"
+	stop
+endif
+allocate(var(n1,n2,n3,7), x1(n1), x2(n2), x3(n3))
+allocate(stat1(n3,7), stat2(n3,7), corr(n3,9))
+do k=1,n3
+	read(10,*)
+	do j=1,n2
+		read(10,*)
+		do i=1,n1
+			read(10,'(3(1X,F10.6),7(1X,E22.15))',iostat=st) x1(i),x2(j),x3(k),var(i,j,k,1),var(i,j,k,2), &
+				var(i,j,k,3),var(i,j,k,4),var(i,j,k,5),var(i,j,k,6),var(i,j,k,7)
+			if (st /= 0) then
+				if (resOp == 2) then
+					write(6,'(A,5(1X,I4))') "Error in reading component input file at",0,i,j,k,st
+					write(6,'(A)') "Change in MPI process topology since last run could possibly be a reason"
+				else
+					write(6,'(A,4(1X,I4))') "Error in reading combined input file at",i,j,k,st
+				endif
+				stop
+			endif
+		enddo
+	enddo
+enddo
+close(10)
+write(6,'(A)') "Input file successfully read"
+
+!--------------------------- device instance -----------------------------
+prm%n1 = n1; prm%n2 = n2; prm%n3 = n3
+prm%istat = istat; prm%precision = prec; prm%reserved = 0
+prm%Re = Re; prm%Mac = Mac; prm%dt = dt
+prm%PexVal = PexVal; prm%PeyVal = PeyVal; prm%PezVal = PezVal
+prm%Ri_b = 0.0d0; prm%slip_len = 0.0d0
+rc = cc_create(prm, h)
+if (rc /= 0) call fatal("cc_create")
+rc = cc_upload(h, var, int(nstep, c_int), time)
+if (rc /= 0) call fatal("cc_upload")
+
+!------------------------------- time loop -------------------------------
+call system_clock(clk0, crate)
+do while (nstep < maxstep)
+	! advance to the next step at which the reference does any output
+	nnext = min(maxstep, (nstep/10 + 1)*10, (nstep/istat + 1)*istat, (nstep/nwrite + 1)*nwrite)
+	nsub = nnext - nstep
+	if (mod(nstep,10) == 0) call system_clock(clk0)
+	rc = cc_step(h, int(nsub, c_int))
+	if (rc < 0) call fatal("cc_step")
+	isBlown = rc > 0
+	if (isBlown) nsub = rc - nstep
+	do i = 1, nsub
+		nstep = nstep + 1
+		time = time + dt
+	enddo
+
+	!----------------------- check for unphysical values ----------------------
+	if (isBlown) then
+		rc = cc_download(h, var, ns_c, t_c)
+		if (rc /= 0) call fatal("cc_download")
+		do k=1,n3
+			do j=1,n2
+				do i=1,n1
+					if (var(i,j,k,1) < 0.0d0 .or. var(i,j,k,6) < 0.0d0) then
+						write(6,102) nstep,0,i,j,k,var(i,j,k,1),var(i,j,k,6)
+					endif
+				enddo
+			enddo
+		enddo
+		open(unit=10, file="../debug/3D001.dat")
+		write(10,103) 'TITLE ="',time,nstep,'"'
+		write(10,*) 'Variables ="x","y","z","Rho","U","V","W","E","T","P"'
+		write(10,104) 'ZONE k=',n3,',j=',n2,',i=',n1,',DATAPACKING="POINT"'
+		do k=1,n3
+			write(10,*)
+			do j=1,n2
+				write(10,*)
+				do i=1,n1
+					write(10,116) x1(i),x2(j),x3(k),var(i,j,k,1),var(i,j,k,2),var(i,j,k,3),var(i,j,k,4), &
+						var(i,j,k,5),var(i,j,k,6),var(i,j,k,7)
+				enddo
+			enddo
+		enddo
+		close(10)
+		call cc_destroy(h)
+		stop
+	endif
+
+	!--------------------------- solution file ---------------------------
+	if (mod(nstep,nwrite) == 0) then
+		rc = cc_download(h, var, ns_c, t_c)
+		if (rc /= 0) call fatal("cc_download")
+		if (mod(nstep/nwrite,2) == 0) then
+			filename = "../output/3D001.dat"
+		else
+			filename = "../output/3D001_.dat"
+		endif
+		open(unit=10, file=trim(filename))
+		write(10,103) 'TITLE ="',time,nstep,'"'
+		write(10,*) 'Variables ="x","y","z","Rho","U","V","W","E","T","P"'
+		write(10,104) 'ZONE k=',n3,',j=',n2,',i=',n1,',DATAPACKING="POINT"'
+		do k=1,n3
+			write(10,*)
+			do j=1,n2
+				write(10,*)
+				do i=1,n1
+					write(10,101) x1(i),x2(j),x3(k),var(i,j,k,1),var(i,j,k,2),var(i,j,k,3),var(i,j,k,4), &
+						var(i,j,k,5),var(i,j,k,6),var(i,j,k,7)
+				enddo
+			enddo
+		enddo
+		close(10)
+		open(unit=10, file="../output/domInfo.txt")
+		write(10,105) n1, n2, n3
+		write(10,106) 1
+		write(10,107) 1, n1, 1, n2, 1, n3
+		close(10)
+	endif
+
+	!------------------------- bulk diagnostics --------------------------
+	if (mod(nstep,10) == 0) then
+		rc = cc_diag(h, x3, x2(n2), dg)
+		if (rc /= 0) call fatal("cc_diag")
+		open(10, file="../output/KE.dat", access="append")
+		write(10,108) time, dg(1)
+		close(10)
+		open(10, file="../output/rhobulk.dat", access="append")
+		write(10,109) time, dg(2:6)
+		close(10)
+		open(10, file="../output/Ubulk.dat", access="append")
+		write(10,109) time, dg(7:11)
+		close(10)
+		open(10, file="../output/massflow.dat", access="append")
+		write(10,109) time, dg(12:16)
+		close(10)
+	endif
+
+	!---------------------- wall shear and moments -----------------------
+	if (mod(nstep,istat) == 0) then
+		rc = cc_stats(h, tau, stat1, stat2, corr)
+		if (rc /= 0) call fatal("cc_stats")
+		open(10, file="../output/tauWall.dat", access="append")
+		write(10,110) time, tau(1), tau(2)
+		close(10)
+		open(unit=10, file="../output/moment1st.dat", access="append")
+		write(10,111,advance="no") time
+		do k=1,n3
+			write(10,112,advance="no") stat1(k,1),stat1(k,2),stat1(k,3),stat1(k,4),stat1(k,5),stat1(k,6),stat1(k,7)
+		enddo
+		write(10,*)
+		close(10)
+		open(unit=10, file="../output/moment2nd.dat", access="append")
+		write(10,111,advance="no") time
+		do k=1,n3
+			write(10,112,advance="no") stat2(k,1),stat2(k,2),stat2(k,3),stat2(k,4),stat2(k,5),stat2(k,6),stat2(k,7)
+		enddo
+		write(10,*)
+		close(10)
+		open(unit=10, file="../output/momentJoint.dat", access="append")
+		write(10,111,advance="no") time
+		do k=1,n3
+			write(10,113,advance="no") corr(k,1),corr(k,2),corr(k,3),corr(k,4),corr(k,5),corr(k,6), &
+				corr(k,7),corr(k,8),corr(k,9)
+		enddo
+		write(10,*)
+		close(10)
+	endif
+
+	if (mod(nstep,10) == 0) then
+		call system_clock(clk1)
+		tStart = 0.0d0
+		tEnd = dble(clk1 - clk0)/dble(crate)
+		write(6,114) nstep,time," | CPU Time:",tEnd-tStart,"sec for 10 iters | ", stamp()
+		write(6,'(A)')"-----------------------------------------------------------------------------------"
+	endif
+enddo
+
+call cc_destroy(h)
+write(*,*) "Compressible Channel Flow Solver exited at :",stamp()
+
+101 format(3(1X,F10.6),7(1X,E22.15))
+102 format(I10,4(I4),2(F8.4))
+103 format(A,F20.10,I9,A)
+104 format(A,I3,A,I3,A,I3,A)
+105 format(3(i4))
+106 format(i3)
+107 format(6(i4,1x))
+108 format(2(1X,F15.7))
+109 format(6(1X,F15.7))
+110 format(3(1X,F15.7))
+111 format(1(1X,F15.7))
+112 format(7(1X,F15.7))
+113 format(9(1X,F15.7))
+114 format(I10,F10.5,A,F6.3,2(1X,A))
+116 format(3(1X,F7.4),7(1X,E12.5))
+
+contains
+
+subroutine fatal(where)
+	character(len=*), intent(in) :: where
+	write(6,'(4A)') "Error in ", where, ": ", cc_errmsg()
+	stop 1
+end subroutine fatal
+
+! wall-clock stamp "d Mon yyyy  hh:mm:ss.mmm AM|PM" (the reference's layout)
+function stamp() result(s)
+	character(len=30) :: s
+	character(len=3), parameter :: mon(12) = ['Jan','Feb','Mar','Apr','May','Jun', &
+		'Jul','Aug','Sep','Oct','Nov','Dec']
+	integer :: v(8), hh
+	character(len=3) :: ap
+	call date_and_time(values=v)
+	hh = v(5)
+	ap = 'AM'
+	if (hh >= 12) ap = 'PM'
+	if (hh > 12) hh = hh - 12
+	write(s,'(i2,1x,a,1x,i4,2x,i2,a1,i2.2,a1,i2.2,a1,i3.3,1x,a)') v(3), mon(v(2)), v(1), hh, ':', &
+		v(6), ':', v(7), '.', v(8), trim(ap)
+end function stamp
+
+end program chanCompr

@@ -24,6 +24,7 @@ struct orc {
   int istat;
   double Pi, dx, dy, dXi, invdx, invdy, invdXi;
   double *x1, *x2, *x3, *x3f, *Xi, *jac, *invJac;
+  double x2end;  /* x2(n2) as read from the solution file */
   double dz1, dzN;
   double f1d2c0, f1d2c1, f1d2c2, b1d2c0, b1d2c1, b1d2c2;
   double f2d1c0, f2d1c1, f2d1c2, b2d1c0, b2d1c1, b2d1c2;
@@ -108,6 +109,7 @@ orc_t *orc_create(int n1, int n2, int n3, double Re, double Mac, double dt,
   }
   for (int k = 1; k < n3; k++) o->invJac[k] = 1.0 / o->jac[k]; /* :110 */
   memcpy(o->x3f, o->x3, n3 * sizeof(double));
+  o->x2end = o->x2[n2 - 1];
 
   /* one-sided coefficients, main:59-83 */
   const double *x3 = o->x3;
@@ -171,8 +173,9 @@ void orc_destroy(orc_t *o) {
   free(o);
 }
 
-void orc_set_x3_file(orc_t *o, const double *x3) {
+void orc_set_x3_file(orc_t *o, const double *x3, double x2end) {
   memcpy(o->x3f, x3, (size_t)o->n3 * sizeof(double));
+  o->x2end = x2end;
 }
 
 void orc_set_forcing(orc_t *o, double tauL, double tauU, const double *rb) {
@@ -638,7 +641,7 @@ void orc_diag(const orc_t *o, double *out) {
       }
   out[0] = s / (double)(n1 * n2 * n3);
   const double *x3 = o->x3f;
-  double Ayz = x3[n3 - 1] * o->x2[n2 - 1];
+  double Ayz = x3[n3 - 1] * o->x2end;
   for (int q = 0; q < 3; q++)
     for (int m = 0; m < 5; m++) {
       int p = plane[m];

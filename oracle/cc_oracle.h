@@ -33,9 +33,10 @@ void orc_destroy(orc_t *o);
  * conserved variables, mu, kt (:284-294) and the first halo fill (:297-313). */
 int orc_load(orc_t *o, const double *var, int nstep, double time);
 
-/* x3 as read back from the solution file (F10.6); only the bulk diagnostics
- * read it (main_v1.9.f90:841, 850).  Defaults to the analytic mesh. */
-void orc_set_x3_file(orc_t *o, const double *x3);
+/* x3 and x2(n2) as read back from the solution file (F10.6): the reader
+ * overwrites the analytic mesh with them (main_v1.9.f90:238) and only the
+ * bulk diagnostics use them (:821-830).  Default: the analytic mesh. */
+void orc_set_x3_file(orc_t *o, const double *x3, double x2end);
 
 /* Forcing state (tauxzL, tauxzU, stat1Spc(:,1)); zero until the first
  * statistics step, exactly as the reference's static initialisation. */

@@ -37,7 +37,7 @@ def lib():
                                  C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_load.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double]
-        L.orc_set_x3_file.argtypes = [C.c_void_p, _dp]
+        L.orc_set_x3_file.argtypes = [C.c_void_p, _dp, C.c_double]
         L.orc_set_forcing.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p]
         L.orc_stage.argtypes = [C.c_void_p, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
@@ -84,12 +84,13 @@ class Oracle:
         n1, n2, n3 = self.n
         return (n3, n2, n1)
 
-    def load(self, var, nstep=0, time=0.0, x3_file=None):
+    def load(self, var, nstep=0, time=0.0, x3_file=None, x2_end=None):
         var = np.ascontiguousarray(var, dtype=np.float64)
         assert var.shape == (7,) + self.shape
         self.L.orc_load(self.h, var, nstep, time)
         if x3_file is not None:
-            self.L.orc_set_x3_file(self.h, np.ascontiguousarray(x3_file, dtype=np.float64))
+            self.L.orc_set_x3_file(self.h, np.ascontiguousarray(x3_file, dtype=np.float64),
+                                   float(x2_end))
 
     def set_forcing(self, tauL, tauU, rhobar=None):
         if rhobar is None:

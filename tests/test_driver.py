@@ -1,0 +1,83 @@
+"""The drop-in Fortran driver (compchan3dmpi_amd/fortran/chanCompr.x, calling
+the HIP step through ISO_C_BINDING) against the reference's own output files
+for config 1 (golden fixtures written by the reference build)."""
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from compchan3dmpi_amd import formats
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "compchan3dmpi_amd", "fortran", "chanCompr.x")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def make_run(n1, n2, n3, dt, istat):
+    root = tempfile.mkdtemp(prefix="ccdrv_")
+    var = formats.round_e22_15(formats.laminar_ic(n1, n2, n3))
+    formats.write_run_dir(root, formats.Params(dt=dt, istat=istat), var)
+    return root
+
+
+def run_driver(root, env_extra, timeout=180):
+    env = dict(os.environ)
+    env.update(env_extra)
+    return subprocess.run([EXE], cwd=os.path.join(root, "codes"), capture_output=True, text=True,
+                          timeout=timeout, env=env)
+
+
+def rows(path):
+    return np.array([np.array(l.split(), dtype=np.float64) for l in open(path) if l.strip()])
+
+
+def test_driver_builds_and_links_the_hip_library():
+    assert os.path.exists(EXE)
+    out = subprocess.run(["ldd", EXE], capture_output=True, text=True).stdout
+    assert "libcc_hip.so" in out
+
+
+def test_driver_reports_missing_device():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    root = make_run(12, 8, 16, 0.001, 5)
+    try:
+        p = run_driver(root, {"CC_MAXSTEP": "10"})
+        assert "Input file successfully read" in p.stdout
+        assert "Error in cc_create" in p.stdout
+        assert p.returncode != 0
+    finally:
+        shutil.rmtree(root)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp64-strict"])
+def test_driver_config1_matches_reference_outputs(precision):
+    print(f"\n[case] driver c1 100 steps {precision}", flush=True)
+    g = np.load(os.path.join(GOLD, "c1_64x32x48_s100.npz"))
+    root = make_run(64, 32, 48, 0.0015, 10)
+    try:
+        p = run_driver(root, {"CC_MAXSTEP": "100", "CC_NWRITE": "100", "CC_PRECISION": precision})
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        assert p.stdout.count("sec for 10 iters") == 10
+        out = os.path.join(root, "output")
+        # solution file at step 100 (E22.15): the reference's state to 1e-12
+        t, ns, x1, x2, x3, v = formats.read_3d(os.path.join(out, "3D001_.dat"))
+        assert ns == 100 and abs(t - 0.15) < 1e-9
+        for c in range(7):
+            ref = g["plane_k24"][c]
+            assert np.linalg.norm(v[c, 23] - ref) <= 1e-12 * np.linalg.norm(ref), c
+        assert np.allclose(v.reshape(7, -1).sum(axis=1), g["sums"], rtol=1e-12, atol=1e-9)
+        # appended diagnostics, F15.7: equal to the reference's text to the last digit
+        for name in ("tauWall", "KE", "rhobulk", "Ubulk", "massflow", "moment1st"):
+            mine = rows(os.path.join(out, name + ".dat"))
+            ref = g[name]
+            assert mine.shape == ref.shape, name
+            assert np.abs(mine - ref).max() <= 1.01e-7, name
+        dom = open(os.path.join(out, "domInfo.txt")).read().split()
+        assert dom == ["64", "32", "48", "1", "1", "64", "1", "32", "1", "48"]
+    finally:
+        shutil.rmtree(root)

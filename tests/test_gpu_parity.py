@@ -126,3 +126,23 @@ def test_c1_mixed_precision_error():
     print(f"  mean-u profile rel error = {e:.3e}", flush=True)
     assert e <= 1e-5
     assert relL2(conserved(vg), conserved(vo)) <= 1e-4
+
+
+def test_bulk_diagnostics_match_oracle():
+    """KE, bulk density, velocity and mass flow (main_v1.9.f90:799-906) on
+    the device against the oracle, with the file-rounded mesh."""
+    banner("bulk diagnostics 64x32x48 after 10 steps")
+    n1, n2, n3 = 64, 32, 48
+    v0 = ic(n1, n2, n3)
+    x1, x2, x3 = formats.mesh(n1, n2, n3)
+    x3f = np.round(x3, 6)
+    x2e = round(x2[-1], 6)
+    o = orc.Oracle(n1, n2, n3, dt=0.0015, istat=10)
+    o.load(v0, x3_file=x3f, x2_end=x2e)
+    o.step(10)
+    for prec in ("fp64-strict", "fp64"):
+        ch = Channel(n1, n2, n3, dt=0.0015, istat=10, precision=prec)
+        ch.upload(v0)
+        ch.step(10)
+        d = ch.diag(x3f, x2e)
+        assert np.allclose(d, o.diag(), rtol=1e-12, atol=1e-15), prec
