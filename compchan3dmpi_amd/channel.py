@@ -25,7 +25,9 @@ PRECISIONS = {"fp64-strict": FP64_STRICT, "fp64": FP64, "mixed": MIXED}
 
 # Every function the C ABI exports (include/cc_abi.h).
 ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
-               "cc_blown", "cc_profile", "cc_destroy", "cc_last_error", "cc_abi_version")
+               "cc_blown", "cc_profile", "cc_destroy", "cc_last_error", "cc_abi_version",
+               "cc_batch_create", "cc_batch_member", "cc_batch_step", "cc_batch_tune",
+               "cc_batch_destroy")
 
 
 class cc_params(C.Structure):
@@ -68,9 +70,17 @@ def load_library():
     L.cc_profile.argtypes = [C.c_void_p, C.c_int, _dp]
     L.cc_destroy.argtypes = [C.c_void_p]
     L.cc_destroy.restype = None
+    L.cc_batch_create.argtypes = [C.POINTER(cc_params), C.c_int, C.POINTER(C.c_void_p)]
+    L.cc_batch_member.argtypes = [C.c_void_p, C.c_int]
+    L.cc_batch_member.restype = C.c_void_p
+    L.cc_batch_step.argtypes = [C.c_void_p, C.c_int]
+    L.cc_batch_tune.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.cc_batch_destroy.argtypes = [C.c_void_p]
+    L.cc_batch_destroy.restype = None
     L.cc_last_error.restype = C.c_char_p
     for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
-              "cc_blown", "cc_profile", "cc_abi_version"):
+              "cc_blown", "cc_profile", "cc_abi_version", "cc_batch_create", "cc_batch_step",
+              "cc_batch_tune"):
         getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -82,6 +92,13 @@ def _check(L, rc):
     return rc
 
 
+def _params(n1, n2, n3, Re=3000.0, Mac=1.5, dt=1e-4, istat=10, PexVal=2.0, PeyVal=2.0,
+            PezVal=0.2, precision="fp64", Ri_b=0.0, slip_len=0.0):
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    return cc_params(n1, n2, n3, istat, prec, 0, Re, Mac, dt, PexVal, PeyVal, PezVal, Ri_b,
+                     slip_len)
+
+
 class Channel:
     """One channel instance on the current HIP device.
 
@@ -91,18 +108,18 @@ class Channel:
     def __init__(self, n1, n2, n3, Re=3000.0, Mac=1.5, dt=1e-4, istat=10, PexVal=2.0,
                  PeyVal=2.0, PezVal=0.2, precision="fp64", Ri_b=0.0, slip_len=0.0):
         self.L = load_library()
-        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        self.p = cc_params(n1, n2, n3, istat, prec, 0, Re, Mac, dt, PexVal, PeyVal, PezVal, Ri_b,
-                           slip_len)
+        self.p = _params(n1, n2, n3, Re, Mac, dt, istat, PexVal, PeyVal, PezVal, precision, Ri_b,
+                         slip_len)
         self.n = (n1, n2, n3)
+        self.owned = True
         h = C.c_void_p()
         _check(self.L, self.L.cc_create(C.byref(self.p), C.byref(h)))
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "owned", True):
             self.L.cc_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()
@@ -155,3 +172,49 @@ class Channel:
         ijk = (C.c_int * 3)()
         step = self.L.cc_blown(self.h, ijk)
         return step, tuple(ijk)
+
+
+class Member(Channel):
+    """A batch member: the batch owns the handle."""
+
+    def __init__(self, L, h, n, p):
+        self.L = L
+        self.h = C.c_void_p(h)
+        self.n = n
+        self.p = p
+        self.owned = False
+
+
+class Batch:
+    """Several channel instances (each a dict of Channel keyword arguments)
+    advanced together by persistent batched launches."""
+
+    def __init__(self, specs):
+        self.L = load_library()
+        arr = (cc_params * len(specs))()
+        for i, sp in enumerate(specs):
+            arr[i] = _params(**sp)
+        h = C.c_void_p()
+        _check(self.L, self.L.cc_batch_create(arr, len(specs), C.byref(h)))
+        self.h = h
+        self.members = [Member(self.L, self.L.cc_batch_member(h, i),
+                               (sp["n1"], sp["n2"], sp["n3"]), arr[i])
+                        for i, sp in enumerate(specs)]
+
+    def step(self, nsteps=1):
+        return _check(self.L, self.L.cc_batch_step(self.h, nsteps))
+
+    def tune(self, trial_steps=3):
+        out = np.zeros(4)
+        _check(self.L, self.L.cc_batch_tune(self.h, trial_steps, out))
+        return {"ms_per_step": out[0], "tile": (int(out[1]), int(out[2])), "kchunk": int(out[3])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            self.L.cc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -6,6 +6,7 @@
 // kernels, and copies.  Every per-node computation runs on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -41,6 +42,7 @@ struct Chan {
   int n1 = 0, n2 = 0, n3 = 0;
   hipStream_t s = nullptr;
   double *UA = nullptr, *UB = nullptr, *W = nullptr, *rw = nullptr, *pw = nullptr;
+  double *uwall = nullptr, *vwall = nullptr;  // wall slip velocities [2][plane]
   double *jac = nullptr, *invJac = nullptr, *force = nullptr, *consts = nullptr;
   double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr, *dbuf = nullptr;
   int* halt = nullptr;
@@ -51,7 +53,9 @@ struct Chan {
   int blown_step = 0;
   int blown_ijk[3] = {0, 0, 0};
   std::vector<double> x3;
-  FusedCfg cfg{32, 16, 0};
+  Inst inst{};      // descriptor of this instance for the fused kernels
+  int variant = 0;  // tile shape of the stage kernel
+  int kchunk = 0;   // planes per block (0 = automatic)
   // per-launch timing of the stage's main kernel (cc_profile)
   bool prof = false;
   std::vector<hipEvent_t> evpool;
@@ -59,8 +63,9 @@ struct Chan {
   long long prof_n = 0;
 
   WallP wallp() const {
-    if (prm.precision == CC_FP64_STRICT) return WallP{W + 5 * g.field, (long long)(n3 - 1) * g.plane};
-    return WallP{pw, g.plane};
+    if (prm.precision == CC_FP64_STRICT)
+      return WallP{W + 5 * g.field, (long long)(n3 - 1) * g.plane, nullptr, nullptr};
+    return WallP{pw, g.plane, uwall, vwall};
   }
 };
 
@@ -118,13 +123,16 @@ void setup_mesh(Chan& c) {
   ph.Pey = c.prm.PeyVal;
   ph.Pez = c.prm.PezVal;
   ph.gkRP = gama * ph.invRe * kInvPr;
+  ph.Ri = c.prm.Ri_b;
+  ph.Ls = c.prm.slip_len;
   (void)hipMemcpy(c.jac, jac.data(), n3 * sizeof(double), hipMemcpyHostToDevice);
   (void)hipMemcpy(c.invJac, invJac.data(), n3 * sizeof(double), hipMemcpyHostToDevice);
 }
 
 void free_chan(Chan* c) {
   if (!c) return;
-  double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->jac, &c->invJac, &c->force,
+  double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->uwall, &c->vwall, &c->jac,
+                     &c->invJac, &c->force,
                      &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red, &c->dbuf};
   for (double** b : bufs)
     if (*b) (void)hipFree(*b);
@@ -133,6 +141,75 @@ void free_chan(Chan* c) {
   if (c->blow) (void)hipFree(c->blow);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
+}
+
+
+// ---- batched ensembles -------------------------------------------------------
+struct Batch {
+  std::vector<Chan*> m;
+  int mixed = 0;
+  hipStream_t s = nullptr;
+  Inst* d_insts = nullptr;
+  int4* d_work = nullptr;
+  int nwork = 0, cap = 0;
+  unsigned* counter = nullptr;
+  unsigned base = 0;
+  int variant = 0, kchunk = 0, nblocks = 0, ncu = 0;
+  int maxP1 = 0, maxP2 = 0;
+};
+
+// Work list for the current tile shape: every (tile, z-chunk) of every
+// instance, largest instances first so that the small ones fill the tail.
+int build_work(Batch* b) {
+  const int n = (int)b->m.size();
+  std::vector<Inst> insts(n);
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) {
+    Chan* c = b->m[i];
+    c->variant = b->variant;
+    c->kchunk = b->kchunk;
+    set_tiling(c->inst, b->variant, b->kchunk);
+    insts[i] = c->inst;
+    order[i] = i;
+  }
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    if (b->m[x]->g.field != b->m[y]->g.field) return b->m[x]->g.field > b->m[y]->g.field;
+    return x < y;
+  });
+  std::vector<int4> work;
+  for (int i : order) {
+    const Inst& a = insts[i];
+    for (int zc = 0; zc < a.nz; zc++)
+      for (int ty = 0; ty < a.nty; ty++)
+        for (int tx = 0; tx < a.ntx; tx++) work.push_back(make_int4(i, tx, ty, zc));
+  }
+  b->nwork = (int)work.size();
+  if (b->nwork > b->cap) {
+    if (b->d_work) HIPCHK(hipFree(b->d_work));
+    HIPCHK(hipMalloc((void**)&b->d_work, b->nwork * sizeof(int4)));
+    b->cap = b->nwork;
+  }
+  HIPCHK(hipMemcpy(b->d_work, work.data(), b->nwork * sizeof(int4), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_insts, insts.data(), n * sizeof(Inst), hipMemcpyHostToDevice));
+  b->nblocks = b->ncu * batched_blocks_per_cu(b->mixed, b->variant);
+  return 0;
+}
+
+// One batched time step at step number `step` (stats included when due).
+void batch_enqueue_step(Batch* b, int step, bool stats, hipEvent_t e0, hipEvent_t e1,
+                        hipEvent_t e2, hipEvent_t e3) {
+  const int n = (int)b->m.size();
+  for (int st = 1; st <= 2; st++) {
+    launch_stage_batched(b->d_insts, b->d_work, b->nwork, b->counter, b->base, b->nblocks,
+                         b->mixed, b->variant, st, step, n, b->maxP1, b->maxP2, b->s,
+                         st == 1 ? e0 : e2, st == 1 ? e1 : e3);
+    b->base += (unsigned)(b->nwork + b->nblocks);
+  }
+  if (stats)
+    for (Chan* c : b->m)
+      if (step % c->prm.istat == 0)
+        launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
+                     c->stats, c->halt, step, b->s);
 }
 
 }  // namespace
@@ -173,7 +250,8 @@ int cc_create(const cc_params* p, void** handle) {
   auto al = [&](double** b, size_t n) { return hipMalloc((void**)b, n * sizeof(double)); };
   if (hipStreamCreate(&c->s) != hipSuccess || al(&c->UA, 5 * F) != hipSuccess ||
       al(&c->UB, 5 * F) != hipSuccess || al(&c->rw, 2 * PL) != hipSuccess ||
-      al(&c->pw, 2 * PL) != hipSuccess || al(&c->jac, n3) != hipSuccess ||
+      al(&c->pw, 2 * PL) != hipSuccess || al(&c->uwall, 2 * PL) != hipSuccess ||
+      al(&c->vwall, 2 * PL) != hipSuccess || al(&c->jac, n3) != hipSuccess ||
       al(&c->invJac, n3) != hipSuccess || al(&c->force, 2 + n3) != hipSuccess ||
       al(&c->consts, kNStat * n3) != hipSuccess || al(&c->stats, kNStat * n3 + 2) != hipSuccess ||
       al(&c->Vfile, 7 * full) != hipSuccess || al(&c->red, 4096) != hipSuccess ||
@@ -184,6 +262,25 @@ int cc_create(const cc_params* p, void** handle) {
     return fail(CC_ENOMEM, "device allocation failed");
   }
   setup_mesh(*c);
+  {
+    Inst& a = c->inst;
+    a.g = c->g;
+    a.ph = c->ph;
+    a.UA = c->UA;
+    a.UB = c->UB;
+    a.pw = c->pw;
+    a.rw = c->rw;
+    a.uw = c->uwall;
+    a.vw = c->vwall;
+    a.jac = c->jac;
+    a.invJac = c->invJac;
+    a.force = c->force;
+    a.halt = c->halt;
+    a.blow = c->blow;
+    a.n1 = c->n1;
+    a.n2 = c->n2;
+    set_tiling(a, c->variant, c->kchunk);
+  }
   if (hipMemset(c->force, 0, (2 + n3) * sizeof(double)) != hipSuccess ||
       hipMemset(c->stats, 0, (kNStat * n3 + 2) * sizeof(double)) != hipSuccess) {
     free_chan(c);
@@ -204,6 +301,10 @@ int cc_upload(void* handle, const double* var, int nstep, double time) {
   // forcing state: zero until the first statistics step (the reference's
   // static initialisation of tauxzL, tauxzU, stat1Spc)
   HIPCHK(hipMemsetAsync(c->force, 0, (2 + c->n3) * sizeof(double), c->s));
+  // walls start at rest (the file's wall velocity is zero for every
+  // reference state; the first wall update sets the slip velocity)
+  HIPCHK(hipMemsetAsync(c->uwall, 0, 2 * c->g.plane * sizeof(double), c->s));
+  HIPCHK(hipMemsetAsync(c->vwall, 0, 2 * c->g.plane * sizeof(double), c->s));
   HIPCHK(hipMemsetAsync(c->halt, 0, sizeof(int), c->s));
   const unsigned long long big = ~0ull;
   HIPCHK(hipMemcpyAsync(c->blow, &big, sizeof(big), hipMemcpyHostToDevice, c->s));
@@ -248,12 +349,8 @@ int cc_step(void* handle, int nsteps) {
       return fail(CC_EINVAL, "fused precision modes are not built into this library");
 #else
       const int mixed = c->prm.precision == CC_MIXED;
-      launch_stage_fused(c->g, c->ph, mixed, c->cfg, c->UA, c->UA, c->UB, c->pw, c->rw, c->jac,
-                         c->invJac, c->force, 1, c->halt, c->blow, step, c->n1, c->n2, c->s,
-                         ev(s, 0), ev(s, 1));
-      launch_stage_fused(c->g, c->ph, mixed, c->cfg, c->UB, c->UA, c->UA, c->pw, c->rw, c->jac,
-                         c->invJac, c->force, 2, c->halt, c->blow, step, c->n1, c->n2, c->s,
-                         ev(s, 2), ev(s, 3));
+      launch_stage_fused(c->inst, mixed, c->variant, 1, step, c->s, ev(s, 0), ev(s, 1));
+      launch_stage_fused(c->inst, mixed, c->variant, 2, step, c->s, ev(s, 2), ev(s, 3));
 #endif
     }
     if (step % c->prm.istat == 0)
@@ -375,6 +472,193 @@ int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
   HIPCHK(hipMemcpyAsync(out, dout, 16 * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return 0;
+}
+
+
+int cc_batch_create(const cc_params* p, int n, void** batch) {
+  if (!p || !batch || n < 1) return fail(CC_EINVAL, "null argument or empty batch");
+  *batch = nullptr;
+  for (int i = 0; i < n; i++)
+    if (p[i].precision != p[0].precision || p[i].precision == CC_FP64_STRICT)
+      return fail(CC_EINVAL, "a batch runs one precision mode, FP64 or mixed (not strict)");
+  Batch* b = new Batch();
+  b->mixed = p[0].precision == CC_MIXED;
+  for (int i = 0; i < n; i++) {
+    void* h = nullptr;
+    const int rc = cc_create(&p[i], &h);
+    if (rc) {
+      for (Chan* c : b->m) free_chan(c);
+      delete b;
+      return rc;
+    }
+    Chan* c = (Chan*)h;
+    b->m.push_back(c);
+    b->maxP1 = std::max(b->maxP1, c->g.P1);
+    b->maxP2 = std::max(b->maxP2, c->g.P2);
+  }
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    b->ncu = 256;
+  if (hipStreamCreate(&b->s) != hipSuccess ||
+      hipMalloc((void**)&b->d_insts, n * sizeof(Inst)) != hipSuccess ||
+      hipMalloc((void**)&b->counter, sizeof(unsigned)) != hipSuccess ||
+      hipMemset(b->counter, 0, sizeof(unsigned)) != hipSuccess) {
+    cc_batch_destroy(b);
+    return fail(CC_ENOMEM, "batch allocation failed");
+  }
+  const int rc = build_work(b);
+  if (rc) {
+    cc_batch_destroy(b);
+    return rc;
+  }
+  *batch = b;
+  return 0;
+}
+
+void* cc_batch_member(void* batch, int i) {
+  Batch* b = (Batch*)batch;
+  if (!b || i < 0 || i >= (int)b->m.size()) return nullptr;
+  return b->m[i];
+}
+
+int cc_batch_step(void* batch, int nsteps) {
+  Batch* b = (Batch*)batch;
+  if (!b) return fail(CC_EINVAL, "null batch");
+  if (nsteps < 0) return fail(CC_EINVAL, "negative step count");
+  int step0 = -1;
+  for (Chan* c : b->m) {
+    if (!c->loaded) return fail(CC_ESTATE, "a batch member has no state uploaded");
+    const int ns = c->blown_step ? -1 : c->nstep;
+    if (ns >= 0) {
+      if (step0 >= 0 && ns != step0) return fail(CC_ESTATE, "batch members are at different steps");
+      step0 = ns;
+    }
+  }
+  if (step0 < 0) return fail(CC_ESTATE, "every batch member has stopped");
+  for (int s = 1; s <= nsteps; s++) batch_enqueue_step(b, step0 + s, true, nullptr, nullptr, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->s));
+  int first = 0;
+  for (Chan* c : b->m) {
+    if (c->blown_step) continue;
+    int h = 0;
+    HIPCHK(hipMemcpy(&h, c->halt, sizeof(int), hipMemcpyDeviceToHost));
+    const int upto = h ? h : step0 + nsteps;
+    while (c->nstep < upto) {
+      c->nstep++;
+      c->time = c->time + c->prm.dt;
+    }
+    if (nsteps > 0) c->fresh = false;
+    if (h) {
+      unsigned long long lin = 0;
+      HIPCHK(hipMemcpy(&lin, c->blow, sizeof(lin), hipMemcpyDeviceToHost));
+      c->blown_step = h;
+      c->blown_ijk[0] = (int)(lin % c->n1) + 1;
+      c->blown_ijk[1] = (int)((lin / c->n1) % c->n2) + 1;
+      c->blown_ijk[2] = (int)(lin / ((unsigned long long)c->n1 * c->n2)) + 1;
+      if (!first || h < first) first = h;
+    }
+  }
+  return first;
+}
+
+int cc_batch_tune(void* batch, int trial_steps, double* out) {
+  Batch* b = (Batch*)batch;
+  if (!b || trial_steps < 1) return fail(CC_EINVAL, "bad arguments");
+  for (Chan* c : b->m)
+    if (!c->loaded) return fail(CC_ESTATE, "a batch member has no state uploaded");
+  // snapshot every member's evolving state: trials advance it
+  struct Snap {
+    std::vector<double*> dst;
+    std::vector<size_t> n;
+  };
+  std::vector<Snap> snaps(b->m.size());
+  auto bufs = [](Chan* c, std::vector<std::pair<double*, size_t>>& v) {
+    const size_t F = (size_t)c->g.field, PL = (size_t)c->g.plane;
+    v = {{c->UA, 5 * F}, {c->UB, 5 * F}, {c->pw, 2 * PL}, {c->rw, 2 * PL},
+         {c->uwall, 2 * PL}, {c->vwall, 2 * PL}, {c->force, 2 + (size_t)c->n3}};
+  };
+  for (size_t i = 0; i < b->m.size(); i++) {
+    std::vector<std::pair<double*, size_t>> v;
+    bufs(b->m[i], v);
+    for (auto& pr : v) {
+      double* d = nullptr;
+      HIPCHK(hipMalloc((void**)&d, pr.second * sizeof(double)));
+      HIPCHK(hipMemcpyAsync(d, pr.first, pr.second * sizeof(double), hipMemcpyDeviceToDevice, b->s));
+      snaps[i].dst.push_back(d);
+      snaps[i].n.push_back(pr.second);
+    }
+  }
+  int step0 = 0;
+  for (Chan* c : b->m) step0 = std::max(step0, c->nstep);
+  std::vector<int> variants = b->mixed ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0};
+  const int kcs[] = {0, 8, 16, 32, 64};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bv = b->variant, bk = b->kchunk;
+  for (int v : variants)
+    for (int kc : kcs) {
+      b->variant = v;
+      b->kchunk = kc;
+      if (build_work(b)) continue;
+      batch_enqueue_step(b, step0 + 1, false, nullptr, nullptr, nullptr, nullptr);  // warm-up
+      HIPCHK(hipEventRecord(e0, b->s));
+      for (int t = 0; t < trial_steps; t++)
+        batch_enqueue_step(b, step0 + 2 + t, false, nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(e1, b->s));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      const double per = ms / trial_steps;
+      if (per < best) {
+        best = per;
+        bv = v;
+        bk = kc;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // restore the state and the halt flags the trials may have set
+  for (size_t i = 0; i < b->m.size(); i++) {
+    std::vector<std::pair<double*, size_t>> v;
+    bufs(b->m[i], v);
+    for (size_t k = 0; k < v.size(); k++) {
+      HIPCHK(hipMemcpyAsync(v[k].first, snaps[i].dst[k], v[k].second * sizeof(double),
+                            hipMemcpyDeviceToDevice, b->s));
+    }
+    Chan* c = b->m[i];
+    const int h = c->blown_step;
+    HIPCHK(hipMemcpyAsync(c->halt, &h, sizeof(int), hipMemcpyHostToDevice, b->s));
+    HIPCHK(hipStreamSynchronize(b->s));
+    for (double* d : snaps[i].dst) (void)hipFree(d);
+  }
+  b->variant = bv;
+  b->kchunk = bk;
+  const int rc = build_work(b);
+  if (rc) return rc;
+  if (out) {
+    int tx, ty;
+    tile_of(bv, tx, ty);
+    out[0] = best;
+    out[1] = tx;
+    out[2] = ty;
+    out[3] = b->m[0]->inst.kchunk;
+  }
+  return 0;
+}
+
+void cc_batch_destroy(void* batch) {
+  Batch* b = (Batch*)batch;
+  if (!b) return;
+  for (Chan* c : b->m) free_chan(c);
+  if (b->d_insts) (void)hipFree(b->d_insts);
+  if (b->d_work) (void)hipFree(b->d_work);
+  if (b->counter) (void)hipFree(b->counter);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
 }
 
 void cc_destroy(void* handle) { free_chan((Chan*)handle); }

@@ -47,16 +47,17 @@ __device__ __forceinline__ void stat_terms(const Prim& P, double q[kNStat]) {
   q[22] = P.p * P.T;
 }
 
-// Wall state after the boundary update (main_v1.9.f90:624-639): u=v=w=0,
-// T=1, E=T, rho = gama Ma^2 p / T.
-__device__ __forceinline__ Prim wall_prim(const Phys& ph, double rho, double p) {
+// Wall state after the boundary update (main_v1.9.f90:624-639): u=v=w=0
+// (or the slip velocity), T=1, E=T+ggM/2|u|^2, rho = gama Ma^2 p / T.
+__device__ __forceinline__ Prim wall_prim(const Phys& ph, double rho, double p, double u = 0.0,
+                                          double v = 0.0) {
   Prim P;
   P.rho = rho;
-  P.u = 0.0;
-  P.v = 0.0;
+  P.u = u;
+  P.v = v;
   P.w = 0.0;
   P.T = 1.0;
-  P.E = 1.0 + ph.ggM * 0.5 * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+  P.E = 1.0 + ph.ggM * 0.5 * (u * u + v * v + 0.0 * 0.0);
   P.p = p;
   return P;
 }
@@ -128,7 +129,7 @@ __global__ void k_consts(Geom g, Phys ph, int n1, int n2, const double* __restri
         stat_terms(P, tq);
         for (int q = 0; q < kNStat; q++) acc[q] += tq[q];
       }
-  } else {
+  } else if (ph.Ls == 0.0) {  // slip walls: the ring holds periodic images
     for (int j = 0; j < n2; j++)
       for (int i = 0; i < n1; i++) {
         if (!(i == 0 || i == n1 - 1 || j == 0 || j == n2 - 1)) continue;
@@ -159,7 +160,7 @@ __global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __rest
     for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, n3, i, j, k, c)] = V[fidx(n1, n2, n3, i, j, k, c)];
     return;
   }
-  if (wall && (ei || ej)) {
+  if (wall && (ei || ej) && ph.Ls == 0.0) {
     const double p = V[fidx(n1, n2, n3, i, j, k, 6)];
     P = wall_prim(ph, ph.gM2 * p / 1.0, p);
   } else {
@@ -168,7 +169,9 @@ __global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __rest
     const long long n2d = (long long)jj * g.P1 + ii;
     if (wall) {
       const bool top = (k == n3 - 1);
-      P = wall_prim(ph, rw[(top ? g.plane : 0) + n2d], pw.p[(top ? pw.top_off : 0) + n2d]);
+      const long long wo = (top ? g.plane : 0) + n2d;
+      P = wall_prim(ph, rw[wo], pw.p[(top ? pw.top_off : 0) + n2d], pw.uw ? pw.uw[wo] : 0.0,
+                    pw.vw ? pw.vw[wo] : 0.0);
     } else {
       const long long F = g.field, n = idx(g, ii, jj, k);
       P = prim_of(ph, U[n], U[F + n], U[2 * F + n], U[3 * F + n], U[4 * F + n]);
@@ -202,7 +205,7 @@ __device__ __forceinline__ void block_sum(double (&a)[NV], double* lds /* [nwave
 
 // Wall shear (main_v1.9.f90:929-988): block 0 bottom, block 1 top.
 __global__ void __launch_bounds__(256) k_tauwall(Geom g, Phys ph, int n1, int n2,
-                                                 const double* __restrict__ U,
+                                                 const double* __restrict__ U, WallP pw,
                                                  double* __restrict__ force,
                                                  double* __restrict__ tau_out, const int* halt,
                                                  int step) {
@@ -220,11 +223,12 @@ __global__ void __launch_bounds__(256) k_tauwall(Geom g, Phys ph, int n1, int n2
     const double dwdx = mu * (0.0 - 0.0) * 0.5 * ph.invdx;
     const long long a = idx(g, ii, jj, top ? N - 1 : 1), b = idx(g, ii, jj, top ? N - 2 : 2);
     const double u2 = U[F + a] / U[a], u3 = U[F + b] / U[b];
+    const double u1 = pw.uw ? pw.uw[(top ? g.plane : 0) + n] : 0.0;
     double dudz;
     if (!top)
-      dudz = mu * (ph.f1d2c0 * 0.0 + ph.f1d2c1 * u2 + ph.f1d2c2 * u3);
+      dudz = mu * (ph.f1d2c0 * u1 + ph.f1d2c1 * u2 + ph.f1d2c2 * u3);
     else
-      dudz = mu * (ph.b1d2c0 * 0.0 + ph.b1d2c1 * u2 + ph.b1d2c2 * u3);
+      dudz = mu * (ph.b1d2c0 * u1 + ph.b1d2c1 * u2 + ph.b1d2c2 * u3);
     acc[0] += (dwdx + dudz) * ph.invRe;
   }
   block_sum<1>(acc, lds);
@@ -261,7 +265,11 @@ __global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int
     double wgt = 1.0;
     if (wall) {
       const bool top = kk == n3 - 1;
-      P = wall_prim(ph, rw[(top ? g.plane : 0) + n], pw.p[(top ? pw.top_off : 0) + n]);
+      const long long wo = (top ? g.plane : 0) + n;
+      P = wall_prim(ph, rw[wo], pw.p[(top ? pw.top_off : 0) + n], pw.uw ? pw.uw[wo] : 0.0,
+                    pw.vw ? pw.vw[wo] : 0.0);
+      if (ph.Ls != 0.0)  // periodic-image ring: every duplicate counts
+        wgt = ((ii == 0 || ii == g.P1 - 1) ? 2.0 : 1.0) * ((jj == 0 || jj == g.P2 - 1) ? 2.0 : 1.0);
     } else {
       const long long m = idx(g, ii, jj, kk);
       P = prim_of(ph, U[m], U[F + m], U[2 * F + m], U[3 * F + m], U[4 * F + m]);
@@ -365,7 +373,7 @@ void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U
                   const double* rw, const double* consts, double* force, double* stats,
                   const int* halt, int step, hipStream_t s) {
   // stats layout: [23][n3] plane statistics, then tau[2]
-  hipLaunchKernelGGL(k_tauwall, dim3(2), dim3(256), 0, s, g, ph, n1, n2, U, force,
+  hipLaunchKernelGGL(k_tauwall, dim3(2), dim3(256), 0, s, g, ph, n1, n2, U, pw, force,
                      stats + kNStat * g.n3, halt, step);
   hipLaunchKernelGGL(k_planestats, dim3(g.n3), dim3(256), 0, s, g, ph, n1, n2, U, pw, rw, consts,
                      force, stats, halt, step);

@@ -30,7 +30,9 @@ struct Phys {
   double dz1, dzN;
   double f1d2c0, f1d2c1, f1d2c2, f2d1c0, f2d1c1, f2d1c2;
   double b1d2c0, b1d2c1, b1d2c2, b2d1c0, b2d1c1, b2d1c2;
-  double gkRP;                        // gama*invRe*invPr is NOT folded: kept for fast paths
+  double gkRP;                        // gama*invRe*invPr
+  double Ri;                          // buoyancy: body force -Ri rho along z (0 = reference)
+  double Ls;                          // Navier slip length of both walls (0 = no slip)
 };
 
 constexpr double kGama = 1.4;

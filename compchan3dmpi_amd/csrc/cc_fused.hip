@@ -29,23 +29,6 @@
 namespace cc {
 
 template <typename T>
-struct FArgs {
-  Geom g;
-  Phys ph;
-  const double* __restrict__ Uin;
-  const double* Ubase;
-  double* Uout;
-  const double* __restrict__ pw;  // wall pressure: bottom [plane], top [plane]
-  const double* __restrict__ jac;
-  const double* __restrict__ invJac;
-  const double* __restrict__ force;
-  int stage, kchunk, ntx, nty, nz;
-  const int* halt;
-  unsigned long long* blow;
-  int step, n1, n2;
-};
-
-template <typename T>
 __device__ __forceinline__ T quickT(T ue, T uw, T qmm, T qm, T qc, T qp, T qpp) {
   // face-upwinded QUICK pair (main_v1.9.f90:339-361): the east face takes
   // its upwind-biased value from the side ue blows from, the west face from
@@ -121,7 +104,8 @@ __device__ __forceinline__ float pow07<float>(float x) {
 }
 
 template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
+__device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const int step,
+                                           const int tile_x, const int tile_y, const int zc) {
   constexpr int NT = TX * TY;
   constexpr int RX = TX + 4, RY = TY + 4, NR = RX * RY;
   constexpr int NRING = NR - NT;                // halo-2 ring points
@@ -141,21 +125,16 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
   const Geom g = a.g;
   const Phys& ph = a.ph;
   {
+    // a halt set during this launch holds this step, so every thread of the
+    // block takes the same branch here
     const int h = *a.halt;
-    if (h != 0 && h < a.step) return;
+    if (h != 0 && h < step) return;
   }
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each
-  // XCD walks a contiguous range of tiles and their shared halos stay in
-  // that XCD's L2.
-  const int nblk = a.ntx * a.nty * a.nz;
-  int bid = blockIdx.x;
-  if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
-  const int tile_x = bid % a.ntx;
-  bid /= a.ntx;
-  const int tile_y = bid % a.nty;
-  const int zc = bid / a.nty;
+  const double* __restrict__ Uin = stage == 1 ? a.UA : a.UB;
+  const double* Ubase = a.UA;
+  double* Uout = stage == 1 ? a.UB : a.UA;
   const int i0 = tile_x * TX, j0 = tile_y * TY;
   const int kb = 1 + zc * a.kchunk;
   const int ke = min(kb + a.kchunk, g.n3 - 1);
@@ -235,10 +214,13 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
 #pragma unroll
     for (int c = 0; c < 5; c++) Uo[c] = U[c * F + n];
     if (t == 0 || t == n3 - 1) {
-      u = v = w = T(0);
+      const long long wo = (t == 0 ? 0 : PL) + off;
+      u = T(a.uw[wo]);
+      v = T(a.vw[wo]);
+      w = T(0);
       Tt = T(1);
       mu = T(1);
-      p = T(a.pw[(t == 0 ? 0 : PL) + off]);
+      p = T(a.pw[wo]);
     } else {
       const T r = T(Uo[0]);
       const T ir = T(1) / r;
@@ -263,7 +245,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
     if (have_plane) {
       T u, v, w, Tt, p, mu;
       double Uo[5];
-      prim(a.Uin, o_own, t, u, v, w, Tt, p, mu, Uo);
+      prim(Uin, o_own, t, u, v, w, Tt, p, mu, Uo);
       sl[s][0][r_own] = u;
       sl[s][1][r_own] = v;
       sl[s][2][r_own] = w;
@@ -286,7 +268,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
       const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(a.jac[t]) > T(ph.Pez) * mu;
       zq = (zq << 1) | (qz ? 1u : 0u);
       for (int q = 0; q < nring; q++) {
-        prim(a.Uin, o_ring[q], t, u, v, w, Tt, p, mu, Uo);
+        prim(Uin, o_ring[q], t, u, v, w, Tt, p, mu, Uo);
         const int r = r_ring[q];
         sl[s][0][r] = u;
         sl[s][1][r] = v;
@@ -547,19 +529,23 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
                              (half * invdXi * iJ));
       R[1] += src1;
       R[4] += src4;
+      if (ph.Ri != 0.0) {  // buoyancy -Ri rho along z and its work
+        R[3] += -ph.Ri * Uw[2][0];
+        R[4] += -ph.ggM * ph.Ri * Uw[2][3];
+      }
       if (own_valid) {
         const long long n = (long long)q * PL + o_own;
         double Un[5];
-        if (a.stage == 1) {
+        if (stage == 1) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = Uw[2][c] + 0.5 * ph.dt * R[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; c++) Un[c] = a.Ubase[c * F + n] + ph.dt * R[c];
+          for (int c = 0; c < 5; c++) Un[c] = Ubase[c * F + n] + ph.dt * R[c];
         }
 #pragma unroll
-        for (int c = 0; c < 5; c++) a.Uout[c * F + n] = Un[c];
-        if (a.stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
+        for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
+        if (stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
           const double ir = 1.0 / Un[0];
           const double uu = Un[1] * ir, vv = Un[2] * ir, ws = Un[3] * ir;
           const double Tn = Un[4] * ir - ph.ggM * 0.5 * (uu * uu + vv * vv + ws * ws);
@@ -567,7 +553,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
             const unsigned long long lin =
                 ((unsigned long long)q * a.n2 + (unsigned long long)(gj_own + 1)) * a.n1 + (gi_own + 1);
             atomicMin(a.blow, lin);
-            atomicCAS((int*)a.halt, 0, a.step);
+            atomicCAS(a.halt, 0, step);
           }
         }
       }
@@ -577,21 +563,69 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(FArgs<T> a) {
   }
 }
 
+// One instance: blocks cover (tiles x z-chunks).  Blocks are dealt to the 8
+// XCDs round-robin; renumber them so that each XCD walks a contiguous range
+// of tiles and their shared halos stay in that XCD's L2.
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_stage_fused(Inst a, int stage, int step) {
+  const int nblk = a.ntx * a.nty * a.nz;
+  int bid = blockIdx.x;
+  if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+  const int tile_x = bid % a.ntx;
+  bid /= a.ntx;
+  const int tile_y = bid % a.nty;
+  const int zc = bid / a.nty;
+  stage_tile<T, TX, TY>(a, stage, step, tile_x, tile_y, zc);
+}
+
+// Many instances in one persistent launch: each block pulls work items
+// (instance, tile_x, tile_y, z-chunk) from a counter until the list is
+// exhausted.  The counter only grows: every block ends with exactly one
+// fetch past the list, so after a launch it stands at base + nwork + grid,
+// which the host uses as the next base (no reset between launches).
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_stage_batched(const Inst* __restrict__ insts,
+                                                         const int4* __restrict__ work, int nwork,
+                                                         unsigned* counter, unsigned base,
+                                                         int stage, int step) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)(atomicAdd(counter, 1u) - base);
+    __syncthreads();
+    // the item is block-uniform: keep it (and the descriptor loads) scalar
+    const int w = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (w >= nwork) break;
+    const int4 it = work[w];
+    const int inst = __builtin_amdgcn_readfirstlane(it.x);
+    stage_tile<T, TX, TY>(insts[inst], stage, step, __builtin_amdgcn_readfirstlane(it.y),
+                          __builtin_amdgcn_readfirstlane(it.z), __builtin_amdgcn_readfirstlane(it.w));
+    __syncthreads();
+  }
+}
+
 // Wall values and wall-pressure extrapolation in FP64 for the fused path
-// (main_v1.9.f90:622-716): rw <- gama Ma^2 p_old / T, then the new wall
-// pressure from the z-momentum balance at an impervious wall.
-__global__ void __launch_bounds__(256) k_wall_fast(Geom g, Phys ph, const double* __restrict__ U,
-                                                   double* __restrict__ pw, double* __restrict__ rw,
-                                                   int check, int* halt, unsigned long long* blow,
-                                                   int step, int n1, int n2) {
-  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
-  const int jj = blockIdx.y * blockDim.y + threadIdx.y;
-  const int wall = blockIdx.z;  // 0 bottom, 1 top
+// (main_v1.9.f90:622-716): rw <- gama Ma^2 p_old / T, the wall slip velocity
+// (Navier slip u_w = Ls du/dn with the one-sided coefficients; 0 for
+// no-slip walls), then the new wall pressure from the z-momentum balance at
+// an impervious wall, with the gravity term -Ri rho_w when buoyancy is on.
+__device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, int ii, int jj,
+                                          int wall) {
+  const Geom& g = a.g;
+  const Phys& ph = a.ph;
   if (ii >= g.P1 || jj >= g.P2) return;
   {
-    const int h = *halt;
+    const int h = *a.halt;
     if (h != 0 && h < step) return;
   }
+  const double* __restrict__ U = stage == 1 ? a.UB : a.UA;
+  double* __restrict__ pw = a.pw;
+  double* __restrict__ rw = a.rw;
+  double* __restrict__ uw = a.uw;
+  double* __restrict__ vw = a.vw;
+  const int check = stage == 2, n1 = a.n1, n2 = a.n2;
+  int* halt = a.halt;
+  unsigned long long* blow = a.blow;
   const long long F = g.field, PL = g.plane;
   const int N = g.n3 - 1;
   const long long n2d = (long long)jj * g.P1 + ii;
@@ -607,7 +641,8 @@ __global__ void __launch_bounds__(256) k_wall_fast(Geom g, Phys ph, const double
   const int k1 = wall ? N - 1 : 1, k2 = wall ? N - 2 : 2;
   const double c0 = wall ? ph.b1d2c0 : ph.f1d2c0, c1 = wall ? ph.b1d2c1 : ph.f1d2c1,
                c2 = wall ? ph.b1d2c2 : ph.f1d2c2;
-  const double e1 = wall ? ph.b2d1c1 : ph.f2d1c1, e2 = wall ? ph.b2d1c2 : ph.f2d1c2;
+  const double e0 = wall ? ph.b2d1c0 : ph.f2d1c0, e1 = wall ? ph.b2d1c1 : ph.f2d1c1,
+               e2 = wall ? ph.b2d1c2 : ph.f2d1c2;
   struct P {
     double r, u, v, w, p, mu;
   };
@@ -624,46 +659,67 @@ __global__ void __launch_bounds__(256) k_wall_fast(Geom g, Phys ph, const double
     q.mu = pow(T, 0.7);
     return q;
   };
+  // slip velocity of a wall node from the interior values above it
+  const double sden = wall ? (1.0 + ph.Ls * c0) : (1.0 - ph.Ls * c0);
+  const double ssgn = wall ? -ph.Ls : ph.Ls;
+  auto slip = [&](double a1, double a2) {
+    return ph.Ls == 0.0 ? 0.0 : ssgn * (c1 * a1 + c2 * a2) / sden;
+  };
   const int im = wrapi(ii - 1, g.P1), ip = wrapi(ii + 1, g.P1);
   const int jm = wrapi(jj - 1, g.P2), jp = wrapi(jj + 1, g.P2);
-  // wall node values: u = v = w = 0, mu = 1 -> the k=1 (or n3) terms vanish
   const P xp1 = prim(ip, jj, k1), xp2 = prim(ip, jj, k2), xm1 = prim(im, jj, k1), xm2 = prim(im, jj, k2);
   const P yp1 = prim(ii, jp, k1), yp2 = prim(ii, jp, k2), ym1 = prim(ii, jm, k1), ym2 = prim(ii, jm, k2);
   const P c1p = prim(ii, jj, k1), c2p = prim(ii, jj, k2);
-  const double dudxdz =
-      ((c1 * xp1.mu * xp1.u + c2 * xp2.mu * xp2.u) - (c1 * xm1.mu * xm1.u + c2 * xm2.mu * xm2.u)) *
-      0.5 * ph.invdx;
-  const double dvdydz =
-      ((c1 * yp1.mu * yp1.v + c2 * yp2.mu * yp2.v) - (c1 * ym1.mu * ym1.v + c2 * ym2.mu * ym2.v)) *
-      0.5 * ph.invdy;
-  const double d2wdz2 = e1 * c1p.mu * c1p.w + e2 * c2p.mu * c2p.w;
+  // wall node: u = u_slip, v = v_slip, w = 0, T = 1, mu = 1
+  uw[wall * PL + n2d] = slip(c1p.u, c2p.u);
+  vw[wall * PL + n2d] = slip(c1p.v, c2p.v);
+  const double uwp = slip(xp1.u, xp2.u), uwm = slip(xm1.u, xm2.u);
+  const double vwp = slip(yp1.v, yp2.v), vwm = slip(ym1.v, ym2.v);
+  const double dudxdz = ((c0 * uwp + c1 * xp1.mu * xp1.u + c2 * xp2.mu * xp2.u) -
+                         (c0 * uwm + c1 * xm1.mu * xm1.u + c2 * xm2.mu * xm2.u)) *
+                        0.5 * ph.invdx;
+  const double dvdydz = ((c0 * vwp + c1 * yp1.mu * yp1.v + c2 * yp2.mu * yp2.v) -
+                         (c0 * vwm + c1 * ym1.mu * ym1.v + c2 * ym2.mu * ym2.v)) *
+                        0.5 * ph.invdy;
+  const double d2wdz2 = e0 * 0.0 + e1 * c1p.mu * c1p.w + e2 * c2p.mu * c2p.w;
   const double drhow2dz = c1 * c1p.r * c1p.w * c1p.w + c2 * c2p.r * c2p.w * c2p.w;
-  const double lhs =
-      ph.invRe * (-dudxdz - dvdydz + kTwo3rd * (dudxdz + dvdydz - 2.0 * d2wdz2)) + drhow2dz;
+  double lhs = ph.invRe * (-dudxdz - dvdydz + kTwo3rd * (dudxdz + dvdydz - 2.0 * d2wdz2)) + drhow2dz;
+  if (ph.Ri != 0.0) lhs += ph.Ri * r;
   pw[wall * PL + n2d] = -(lhs + c1 * c1p.p + c2 * c2p.p) / c0;
 }
 
-template <typename T, int TX, int TY>
-static void launch_t(const Geom& g, const Phys& ph, const FusedCfg& cfg, const double* Uin,
-                     const double* Ubase, double* Uout, const double* pw, const double* jac,
-                     const double* invJac, const double* force, int stage, int* halt,
-                     unsigned long long* blow, int step, int n1, int n2, hipStream_t s) {
-  FArgs<T> a;
-  a.g = g;
-  a.ph = ph;
-  a.Uin = Uin;
-  a.Ubase = Ubase;
-  a.Uout = Uout;
-  a.pw = pw;
-  a.jac = jac;
-  a.invJac = invJac;
-  a.force = force;
-  a.stage = stage;
-  a.ntx = (g.P1 + TX - 1) / TX;
-  a.nty = (g.P2 + TY - 1) / TY;
-  const int nint = g.n3 - 2;
-  int kc = cfg.kchunk > 0 ? cfg.kchunk : 0;
-  if (kc == 0) {  // enough blocks to cover the chip at one block per CU
+__global__ void __launch_bounds__(256) k_wall_fast(Inst a, int stage, int step) {
+  wall_node(a, stage, step, blockIdx.x * blockDim.x + threadIdx.x,
+            blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z);
+}
+
+// all instances of a batch: blockIdx.z = 2 * instance + wall
+__global__ void __launch_bounds__(256) k_wall_batched(const Inst* __restrict__ insts, int stage,
+                                                      int step) {
+  wall_node(insts[blockIdx.z >> 1], stage, step, blockIdx.x * blockDim.x + threadIdx.x,
+            blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z & 1);
+}
+
+// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (161 KB of
+// LDS, one block per CU); FP32 also has the smaller shapes the batch
+// autotuner chooses between.
+void tile_of(int variant, int& tx, int& ty) {
+  static const int T[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 8}};
+  const int v = (variant >= 0 && variant < 4) ? variant : 0;
+  tx = T[v][0];
+  ty = T[v][1];
+}
+
+// Fill the tiling fields of an instance for a tile shape and z-chunk
+// (kchunk <= 0: enough blocks to cover the chip at one block per CU).
+void set_tiling(Inst& a, int variant, int kchunk) {
+  int TX, TY;
+  tile_of(variant, TX, TY);
+  a.ntx = (a.g.P1 + TX - 1) / TX;
+  a.nty = (a.g.P2 + TY - 1) / TY;
+  const int nint = a.g.n3 - 2;
+  int kc = kchunk;
+  if (kc <= 0) {
     const int tiles = a.ntx * a.nty;
     const int nzw = (512 + tiles - 1) / tiles;
     kc = (nint + nzw - 1) / nzw;
@@ -671,32 +727,75 @@ static void launch_t(const Geom& g, const Phys& ph, const FusedCfg& cfg, const d
   }
   a.kchunk = kc;
   a.nz = (nint + kc - 1) / kc;
-  a.halt = halt;
-  a.blow = blow;
-  a.step = step;
-  a.n1 = n1;
-  a.n2 = n2;
-  const unsigned nblk = (unsigned)(a.ntx * a.nty * a.nz);
-  hipLaunchKernelGGL((k_stage_fused<T, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, a);
 }
 
-void launch_stage_fused(const Geom& g, const Phys& ph, int mixed, const FusedCfg& cfg,
-                        const double* Uin, const double* Ubase, double* Uout, double* pw,
-                        double* rw, const double* jac, const double* invJac, const double* force,
-                        int stage, int* halt, unsigned long long* blow, int step, int n1, int n2,
+template <typename T, int TX, int TY>
+static void launch_one(const Inst& a, int stage, int step, hipStream_t s) {
+  const unsigned nblk = (unsigned)(a.ntx * a.nty * a.nz);
+  hipLaunchKernelGGL((k_stage_fused<T, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
+}
+
+void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int step,
                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (ev0) (void)hipEventRecord(ev0, s);
-  if (mixed)
-    launch_t<float, 32, 16>(g, ph, cfg, Uin, Ubase, Uout, pw, jac, invJac, force, stage, halt,
-                            blow, step, n1, n2, s);
-  else
-    launch_t<double, 32, 16>(g, ph, cfg, Uin, Ubase, Uout, pw, jac, invJac, force, stage, halt,
-                             blow, step, n1, n2, s);
+  if (!mixed) {
+    launch_one<double, 32, 16>(a, stage, step, s);
+  } else {
+    switch (variant) {
+      case 1: launch_one<float, 32, 8>(a, stage, step, s); break;
+      case 2: launch_one<float, 16, 16>(a, stage, step, s); break;
+      case 3: launch_one<float, 64, 8>(a, stage, step, s); break;
+      default: launch_one<float, 32, 16>(a, stage, step, s); break;
+    }
+  }
   if (ev1) (void)hipEventRecord(ev1, s);
   dim3 blk(64, 4);
-  dim3 grd((g.P1 + 63) / 64, (g.P2 + 3) / 4, 2);
-  hipLaunchKernelGGL(k_wall_fast, grd, blk, 0, s, g, ph, Uout, pw, rw, stage == 2, halt, blow,
-                     step, n1, n2);
+  dim3 grd((a.g.P1 + 63) / 64, (a.g.P2 + 3) / 4, 2);
+  hipLaunchKernelGGL(k_wall_fast, grd, blk, 0, s, a, stage, step);
+}
+
+template <typename T, int TX, int TY>
+static int blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_batched<T, TX, TY>, TX * TY, 0) !=
+      hipSuccess)
+    n = 1;
+  return n > 0 ? n : 1;
+}
+
+int batched_blocks_per_cu(int mixed, int variant) {
+  if (!mixed) return blocks_per_cu<double, 32, 16>();
+  switch (variant) {
+    case 1: return blocks_per_cu<float, 32, 8>();
+    case 2: return blocks_per_cu<float, 16, 16>();
+    case 3: return blocks_per_cu<float, 64, 8>();
+    default: return blocks_per_cu<float, 32, 16>();
+  }
+}
+
+void launch_stage_batched(const Inst* d_insts, const int4* d_work, int nwork, unsigned* counter,
+                          unsigned base, int nblocks, int mixed, int variant, int stage, int step,
+                          int ninst, int maxP1, int maxP2, hipStream_t s, hipEvent_t ev0,
+                          hipEvent_t ev1) {
+  if (ev0) (void)hipEventRecord(ev0, s);
+#define CC_LAUNCH_B(T, TX, TY)                                                                  \
+  hipLaunchKernelGGL((k_stage_batched<T, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts, \
+                     d_work, nwork, counter, base, stage, step)
+  if (!mixed) {
+    CC_LAUNCH_B(double, 32, 16);
+  } else {
+    switch (variant) {
+      case 1: CC_LAUNCH_B(float, 32, 8); break;
+      case 2: CC_LAUNCH_B(float, 16, 16); break;
+      case 3: CC_LAUNCH_B(float, 64, 8); break;
+      default: CC_LAUNCH_B(float, 32, 16); break;
+    }
+  }
+#undef CC_LAUNCH_B
+  if (ev1) (void)hipEventRecord(ev1, s);
+  dim3 blk(64, 4);
+  dim3 grd((maxP1 + 63) / 64, (maxP2 + 3) / 4, 2 * ninst);
+  hipLaunchKernelGGL(k_wall_batched, grd, blk, 0, s, d_insts, stage, step);
 }
 
 }  // namespace cc

@@ -11,9 +11,12 @@ namespace cc {
 constexpr int kNStat = 23;
 
 // Wall pressure storage: bottom wall at pwall[0..plane), top at pwall + top_off.
+// uw, vw: wall slip velocities [2][plane] (NULL: no-slip walls).
 struct WallP {
   double* p;
   long long top_off;
+  const double* uw = nullptr;
+  const double* vw = nullptr;
 };
 
 // ---- cc_strict.hip: reference-order FP64 stage ------------------------------
@@ -27,15 +30,36 @@ void launch_stage_strict(const Geom& g, const Phys& ph, const double* Uin, const
 // the stage's main kernel, for per-launch timing.
 
 // ---- cc_fused.hip: fused streaming stage (FP64 or mixed) --------------------
-struct FusedCfg {
-  int tx, ty;   // tile in ii, jj
-  int kchunk;   // planes per block along z
+// One channel instance as the stage kernels see it.  Stage 1 reads UA and
+// writes UB; stage 2 reads UB, takes UA as the RK base and writes UA.
+struct Inst {
+  Geom g;
+  Phys ph;
+  double* UA;
+  double* UB;
+  double* pw;  // wall pressure [2][plane]
+  double* rw;  // wall density [2][plane]
+  double* uw;  // wall slip velocities [2][plane]
+  double* vw;
+  const double* jac;
+  const double* invJac;
+  const double* force;  // tau_L, tau_U, rho_bar(n3)
+  int* halt;
+  unsigned long long* blow;
+  int n1, n2;
+  int kchunk, ntx, nty, nz;  // tiling of the stage kernel
 };
-void launch_stage_fused(const Geom& g, const Phys& ph, int mixed, const FusedCfg& cfg,
-                        const double* Uin, const double* Ubase, double* Uout, double* pw,
-                        double* rw, const double* jac, const double* invJac, const double* force,
-                        int stage, int* halt, unsigned long long* blow, int step, int n1, int n2,
+// tile variants: 0 = 32x16 (the only FP64 one), 1 = 32x8, 2 = 16x16, 3 = 64x8
+void tile_of(int variant, int& tx, int& ty);
+void set_tiling(Inst& a, int variant, int kchunk);
+void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int step,
                         hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+int batched_blocks_per_cu(int mixed, int variant);
+// d_work: int4 (instance, tile_x, tile_y, z-chunk) items
+void launch_stage_batched(const Inst* d_insts, const int4* d_work, int nwork, unsigned* counter,
+                          unsigned base, int nblocks, int mixed, int variant, int stage, int step,
+                          int ninst, int maxP1, int maxP2, hipStream_t s,
+                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // ---- cc_common.hip ----------------------------------------------------------
 // Vfull: the reference's var(n1,n2,n3,7) in Fortran order on the device.

@@ -99,6 +99,27 @@ int cc_blown(void* handle, int* ijk);
 int cc_profile(void* handle, int mode, double* out);
 
 void cc_destroy(void* handle);
+
+/* ---- batched ensembles (no counterpart in the reference, whose ensembles are
+ * separate runs, v1.9_STWNV/bin/startAll.sh:4-18) ------------------------------
+ * n instances of possibly different grids, Ri_b and slip lengths, advanced
+ * together: each RK stage is one persistent launch that pulls (instance, tile,
+ * z-chunk) work items from a queue.  Every member computes exactly what it
+ * computes alone with cc_step.  All members share one precision mode (FP64 or
+ * mixed). */
+int cc_batch_create(const cc_params* p, int n, void** batch);
+/* Handle of member i, for cc_upload, cc_stats, cc_diag, cc_download and
+ * cc_blown (not cc_step, and not cc_destroy: the batch owns it). */
+void* cc_batch_member(void* batch, int i);
+/* nsteps steps of every member that has not stopped; all must stand at the
+ * same step.  Returns 0, or the earliest step at which a member blew up (that
+ * member stops there, the others go on). */
+int cc_batch_step(void* batch, int nsteps);
+/* Autotuner: times trial_steps batched steps for every compiled tile shape
+ * and z-chunk, keeps the fastest, and restores the members' state.
+ * out[0] = best ms per step, out[1..2] = tile (x, y), out[3] = z-chunk. */
+int cc_batch_tune(void* batch, int trial_steps, double* out);
+void cc_batch_destroy(void* batch);
 const char* cc_last_error(void);
 int cc_abi_version(void);
 

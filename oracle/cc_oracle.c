@@ -38,6 +38,10 @@ struct orc {
   int nstep;
   double time;
   int blown[3];
+  /* physics the reference does not have (0 = reference): bulk Richardson
+   * number of a buoyancy body force -Ri rho along z, and the Navier slip
+   * length of both walls */
+  double Ri, Ls;
 };
 
 static const double one = 1.0, two = 2.0, zer = 0.0, quar = 0.25, haf = 0.5;
@@ -176,6 +180,11 @@ void orc_destroy(orc_t *o) {
 void orc_set_x3_file(orc_t *o, const double *x3, double x2end) {
   memcpy(o->x3f, x3, (size_t)o->n3 * sizeof(double));
   o->x2end = x2end;
+}
+
+void orc_set_ext(orc_t *o, double Ri, double Ls) {
+  o->Ri = Ri;
+  o->Ls = Ls;
 }
 
 void orc_set_forcing(orc_t *o, double tauL, double tauU, const double *rb) {
@@ -425,6 +434,10 @@ static void calcSolV(orc_t *o, int iter) {
         src[2] = zer;
         src[3] = zer;
         src[4] = o->ggM * VR(1, i, j, k) * haf * tdiff * o->stat1[k - 1];
+        if (o->Ri != 0.0) { /* buoyancy: -Ri rho along z and its work */
+          src[3] = -o->Ri * SV(0, 0, i, j, k);
+          src[4] = src[4] - o->ggM * o->Ri * SV(3, 0, i, j, k);
+        }
         for (int c = 0; c < 5; c++) {
           double r = src[c] - (A5(o->F1, c, i, j, k) + A5(o->F2, c, i, j, k) +
                                A5(o->G1, c, i, j, k) + A5(o->G2, c, i, j, k) +
@@ -455,6 +468,25 @@ static void calcPrim(orc_t *o) {
       }
 }
 
+/* Wall-plane halo ring from its periodic images (slip walls only: with
+ * u_w != 0 a stale ring would put a jump at the periodic seam). */
+static void wall_ring_images(orc_t *o, int c0, int c1) {
+  const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
+  for (int kk = 0; kk < 2; kk++) {
+    const int k = kk ? n3 : 1;
+    for (int c = c0; c <= c1; c++) {
+      for (int j = 2; j <= n2 - 1; j++) {
+        VR(c, 1, j, k) = VR(c, n1 - 1, j, k);
+        VR(c, n1, j, k) = VR(c, 2, j, k);
+      }
+      for (int i = 1; i <= n1; i++) {
+        VR(c, i, 1, k) = VR(c, i, n2 - 1, k);
+        VR(c, i, n2, k) = VR(c, i, 2, k);
+      }
+    }
+  }
+}
+
 /* ---- wall values, mu, wall pressure, kt: main:622-720 ------------------- */
 static void calcBC(orc_t *o) {
   const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
@@ -464,12 +496,26 @@ static void calcBC(orc_t *o) {
       for (int kk = 0; kk < 2; kk++) {
         int k = kk ? n3 : 1;
         VR(1, i, j, k) = 0.0; VR(2, i, j, k) = 0.0; VR(3, i, j, k) = 0.0;
+        const int upd = i > 1 && i < n1 && j > 1 && j < n2;
+        if (o->Ls != 0.0 && upd) {
+          /* Navier slip u_w = Ls du/dn with the one-sided coefficients;
+           * Ls -> inf is the zero-gradient wall of main_v1.9_g1.f90:611 */
+          for (int c = 1; c <= 2; c++) {
+            if (!kk)
+              VR(c, i, j, 1) = o->Ls * (o->f1d2c1 * VR(c, i, j, 2) + o->f1d2c2 * VR(c, i, j, 3)) /
+                               (1.0 - o->Ls * o->f1d2c0);
+            else
+              VR(c, i, j, n3) = -o->Ls * (o->b1d2c1 * VR(c, i, j, n3 - 1) + o->b1d2c2 * VR(c, i, j, n3 - 2)) /
+                                (1.0 + o->Ls * o->b1d2c0);
+          }
+        }
         VR(5, i, j, k) = 1.0;
         double u = VR(1, i, j, k), v = VR(2, i, j, k), w = VR(3, i, j, k);
         VR(4, i, j, k) = VR(5, i, j, k) + o->ggM * haf * (u * u + v * v + w * w);
         VR(0, i, j, k) = gM2 * VR(6, i, j, k) / VR(5, i, j, k);
       }
     }
+  if (o->Ls != 0.0) wall_ring_images(o, 0, 6);
   /* mu over bn-1..en+1 = 1..n (x, y) and all k (main:643) */
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
@@ -501,6 +547,7 @@ static void calcBC(orc_t *o) {
                           o->f1d2c1 * VR(0, i, j, 2) * (w2 * w2) +
                           o->f1d2c2 * VR(0, i, j, 3) * (w3 * w3);
         double lhs = invRe * (-dudxdz - dvdydz + TWO3RD * (dudxdz + dvdydz - two * d2wdz2)) + drhow2dz;
+        if (o->Ri != 0.0) lhs = lhs + o->Ri * VR(0, i, j, 1);
         VR(6, i, j, 1) = -(lhs + o->f1d2c1 * VR(6, i, j, 2) + o->f1d2c2 * VR(6, i, j, 3)) / o->f1d2c0;
       }
       /* top, main:683-716 */
@@ -528,9 +575,11 @@ static void calcBC(orc_t *o) {
                           o->b1d2c1 * VR(0, i, j, N - 1) * (w2 * w2) +
                           o->b1d2c2 * VR(0, i, j, N - 2) * (w3 * w3);
         double lhs = invRe * (-dudxdz - dvdydz + TWO3RD * (dudxdz + dvdydz - two * d2wdz2)) + drhow2dz;
+        if (o->Ri != 0.0) lhs = lhs + o->Ri * VR(0, i, j, N);
         VR(6, i, j, N) = -(lhs + o->b1d2c1 * VR(6, i, j, N - 1) + o->b1d2c2 * VR(6, i, j, N - 2)) / o->b1d2c0;
       }
     }
+  if (o->Ls != 0.0) wall_ring_images(o, 6, 6);
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) KT(i, j, k) = pow(VR(5, i, j, k), 0.7);

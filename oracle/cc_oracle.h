@@ -38,6 +38,17 @@ int orc_load(orc_t *o, const double *var, int nstep, double time);
  * bulk diagnostics use them (:821-830).  Default: the analytic mesh. */
 void orc_set_x3_file(orc_t *o, const double *x3, double x2end);
 
+/* Physics outside the reference (both 0 = reference): buoyancy body force
+ * -Ri rho along z (work -ggM Ri rho w in the energy equation, and the
+ * gravity term in the wall-pressure balance), and Navier slip of length Ls
+ * on both walls: u_w = Ls du/dn with the one-sided coefficients of
+ * main_v1.9.f90:59-83, so Ls -> inf is the zero-gradient wall of
+ * v1.9_groov/v1.9_g1/codes/main_v1.9_g1.f90:611-612.  With Ls != 0 the
+ * wall-plane halo ring holds periodic images (a stale u = 0 ring would put a
+ * jump at the periodic seam).  Parity unpinned against the reference (it has
+ * neither); pinned by the closed forms in tests/test_oracle_ext.py. */
+void orc_set_ext(orc_t *o, double Ri, double Ls);
+
 /* Forcing state (tauxzL, tauxzU, stat1Spc(:,1)); zero until the first
  * statistics step, exactly as the reference's static initialisation. */
 void orc_set_forcing(orc_t *o, double tauL, double tauU, const double *rhobar);

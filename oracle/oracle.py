@@ -38,6 +38,7 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_load.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double]
         L.orc_set_x3_file.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_set_ext.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_set_forcing.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p]
         L.orc_stage.argtypes = [C.c_void_p, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
@@ -91,6 +92,9 @@ class Oracle:
         if x3_file is not None:
             self.L.orc_set_x3_file(self.h, np.ascontiguousarray(x3_file, dtype=np.float64),
                                    float(x2_end))
+
+    def set_ext(self, Ri=0.0, Ls=0.0):
+        self.L.orc_set_ext(self.h, Ri, Ls)
 
     def set_forcing(self, tauL, tauU, rhobar=None):
         if rhobar is None:
