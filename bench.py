@@ -29,8 +29,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from compchan3dmpi_amd import formats  # noqa: E402
-from compchan3dmpi_amd.channel import Channel  # noqa: E402
+from compchan3dmpi_amd import ensemble, formats  # noqa: E402
+from compchan3dmpi_amd.channel import Batch, Channel  # noqa: E402
 
 METRIC = "cell-updates/s per RK stage (FP64 vs mixed) + % HBM roofline, 1 MI355X"
 NP = (256, 128, 192)
@@ -44,7 +44,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(var, precision, steps, warmup, profile):
+def run_gpu(var, precision, steps, warmup, profile, keep=False):
     n1, n2, n3 = NP
     ch = Channel(n1, n2, n3, dt=DT, istat=ISTAT, precision=precision)
     ch.upload(var)
@@ -60,8 +60,32 @@ def run_gpu(var, precision, steps, warmup, profile):
         raise RuntimeError(f"{precision}: blow-up at step {rc}")
     kms, kn = ch.profile(0) if profile else (0.0, 0)
     tau = ch.stats()[0]
+    final = ch.download()[0] if keep else None
     ch.close()
-    return (t1 - t0), kms, kn, tau
+    return (t1 - t0), kms, kn, tau, final
+
+
+def run_ensemble(steps, warmup):
+    """The 16-instance seeded ensemble (mixed precision, persistent batched
+    launches, autotuned tiles): returns cell-updates/s per RK stage summed
+    over the members, ms per step and the tuned configuration."""
+    sp = ensemble.spec(16)
+    b = Batch([s for s, _ in sp])
+    for m, (_, v) in zip(b.members, sp):
+        m.upload(v)
+    tuned = b.tune(trial_steps=3)
+    rc = b.step(warmup)
+    if rc:
+        raise RuntimeError(f"ensemble: blow-up at warm-up step {rc}")
+    t0 = time.perf_counter()
+    rc = b.step(steps)
+    t1 = time.perf_counter()
+    if rc:
+        raise RuntimeError(f"ensemble: blow-up at step {rc}")
+    nodes = sum(s["n1"] * s["n2"] * s["n3"] for s, _ in sp)
+    b.close()
+    return {"value": nodes * 2 * steps / (t1 - t0), "ms_per_step": 1000 * (t1 - t0) / steps,
+            "instances": len(sp), "nodes": nodes, "tuned": tuned}
 
 
 def cpu_baseline(var, nsteps):
@@ -94,6 +118,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64-strict", "mixed"])
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU restatement timed for cpu_baseline (0 = skip)")
     args = ap.parse_args()
@@ -109,7 +134,8 @@ def main():
     var = formats.laminar_ic(n1, n2, n3)
 
     log(f"[bench] {args.precision}: warmup {args.warmup}, timed {args.steps} steps")
-    t, kms, kn, tau = run_gpu(var, args.precision, args.steps, args.warmup, profile=True)
+    t, kms, kn, tau, v64 = run_gpu(var, args.precision, args.steps, args.warmup, profile=True,
+                                   keep=not args.no_mixed)
     value = nodes * 2 * args.steps / t
     ms_step = 1000.0 * t / args.steps
     avg_launch_s = (kms / kn) / 1000.0 if kn else float("nan")
@@ -119,12 +145,28 @@ def main():
 
     mixed = None
     if not args.no_mixed and args.precision != "mixed":
-        tm, kmm, knm, taum = run_gpu(var, "mixed", args.steps, args.warmup, profile=True)
+        tm, kmm, knm, taum, vmix = run_gpu(var, "mixed", args.steps, args.warmup, profile=True,
+                                           keep=True)
         vm = nodes * 2 * args.steps / tm
+        names = ("rho", "u", "v", "w", "E", "T", "p")
+        err = {nm: float(np.linalg.norm(vmix[c] - v64[c]) / np.linalg.norm(v64[c]))
+               for c, nm in enumerate(names)}
+        ub64 = v64[1, :, 1:-1, 1:-1].mean(axis=(1, 2))
+        ubm = vmix[1, :, 1:-1, 1:-1].mean(axis=(1, 2))
         mixed = {"value": vm, "ms_per_step": 1000.0 * tm / args.steps,
                  "ratio_vs_fp64": vm / value,
-                 "stage_kernel_ms": (kmm / knm) if knm else None}
+                 "stage_kernel_ms": (kmm / knm) if knm else None,
+                 "rel_l2_vs_fp64_after_steps": err,
+                 "mean_u_profile_rel_err": float(np.abs(ubm - ub64).max() / np.abs(ub64).max()),
+                 "tau_wall": [float(x) for x in taum], "tau_wall_fp64": [float(x) for x in tau]}
+        del vmix
         log(f"[bench] mixed: {1000 * tm / args.steps:.4f} ms/step, ratio {vm / value:.3f}")
+
+    ens = None
+    if not args.no_ensemble:
+        ens = run_ensemble(max(50, args.steps // 5), 5)
+        log(f"[bench] ensemble: {ens['ms_per_step']:.4f} ms/step, {ens['value']:.4e} "
+            f"cell-updates/s/stage over {ens['instances']} instances, tuned {ens['tuned']}")
 
     cpu = None
     if args.cpu_steps > 0:
@@ -152,6 +194,7 @@ def main():
                         "Ma 1.5, Re 3000, RK2, hybrid central/QUICK, seeded perturbed laminar start",
             "nP": list(NP), "dt": DT, "istat": ISTAT, "precision": args.precision,
             "mixed": mixed,
+            "ensemble": ens,
         },
         "roofline": {
             "bound": "hbm",
