@@ -27,7 +27,9 @@ PRECISIONS = {"fp64-strict": FP64_STRICT, "fp64": FP64, "mixed": MIXED}
 ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
                "cc_blown", "cc_profile", "cc_destroy", "cc_last_error", "cc_abi_version",
                "cc_batch_create", "cc_batch_member", "cc_batch_step", "cc_batch_tune",
-               "cc_batch_destroy")
+               "cc_batch_destroy", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
+               "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
+               "cc_mlp_destroy", "cc_mlp_last_error", "cc_filter_samples", "cc_set_closure")
 
 
 class cc_params(C.Structure):
@@ -77,10 +79,25 @@ def load_library():
     L.cc_batch_tune.argtypes = [C.c_void_p, C.c_int, _dp]
     L.cc_batch_destroy.argtypes = [C.c_void_p]
     L.cc_batch_destroy.restype = None
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.cc_mlp_create.argtypes = [C.c_uint, C.POINTER(C.c_void_p)]
+    L.cc_mlp_set_params.argtypes = [C.c_void_p, _fp]
+    L.cc_mlp_get_params.argtypes = [C.c_void_p, _fp]
+    L.cc_mlp_forward.argtypes = [C.c_void_p, _fp, C.c_int, _fp]
+    L.cc_mlp_grad.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.POINTER(C.c_double), _fp]
+    L.cc_mlp_train.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float,
+                               C.c_uint, _dp]
+    L.cc_mlp_destroy.argtypes = [C.c_void_p]
+    L.cc_mlp_destroy.restype = None
+    L.cc_mlp_last_error.restype = C.c_char_p
+    L.cc_filter_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+    L.cc_set_closure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.cc_last_error.restype = C.c_char_p
     for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
               "cc_blown", "cc_profile", "cc_abi_version", "cc_batch_create", "cc_batch_step",
-              "cc_batch_tune"):
+              "cc_batch_tune", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
+              "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
+              "cc_filter_samples", "cc_set_closure"):
         getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -168,6 +185,27 @@ class Channel:
         _check(self.L, self.L.cc_profile(self.h, mode, out))
         return float(out[0]), int(out[1])
 
+    def filter_samples(self):
+        """4x4x4 box-filtered features (N,16) and subgrid targets (N,16) of
+        the current state (float32)."""
+        M = C.c_int()
+        _check(self.L, self.L.cc_filter_samples(self.h, None, None, C.byref(M)))
+        X = np.zeros((M.value, 16), dtype=np.float32)
+        T = np.zeros((M.value, 16), dtype=np.float32)
+        _check(self.L, self.L.cc_filter_samples(self.h, X.ctypes.data_as(C.c_void_p),
+                                                T.ctypes.data_as(C.c_void_p), C.byref(M)))
+        return X, T
+
+    def set_closure(self, mlp, norm):
+        """Attach a trained closure (an Mlp) with its normalisation[40]; None detaches."""
+        if mlp is None:
+            _check(self.L, self.L.cc_set_closure(self.h, None, None))
+            self._closure = None
+            return
+        nv = np.ascontiguousarray(norm, dtype=np.float32)
+        _check(self.L, self.L.cc_set_closure(self.h, mlp.h, nv.ctypes.data_as(C.c_void_p)))
+        self._closure = (mlp, nv)  # keep both alive while attached
+
     def blown(self):
         ijk = (C.c_int * 3)()
         step = self.L.cc_blown(self.h, ijk)
@@ -214,6 +252,63 @@ class Batch:
             for m in self.members:
                 m.h = None
             self.L.cc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _mcheck(L, rc):
+    if rc < 0:
+        raise CCError(f"cc_mlp error {rc}: {L.cc_mlp_last_error().decode()}")
+    return rc
+
+
+class Mlp:
+    """The subgrid-closure network on the device (BF16 MFMA, FP32 master)."""
+
+    def __init__(self, seed=0):
+        self.L = load_library()
+        h = C.c_void_p()
+        _mcheck(self.L, self.L.cc_mlp_create(seed, C.byref(h)))
+        self.h = h
+        self.nparams = self.L.cc_mlp_nparams()
+
+    @property
+    def params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        _mcheck(self.L, self.L.cc_mlp_get_params(self.h, out))
+        return out
+
+    @params.setter
+    def params(self, p):
+        _mcheck(self.L, self.L.cc_mlp_set_params(self.h, np.ascontiguousarray(p, dtype=np.float32)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty_like(X)
+        _mcheck(self.L, self.L.cc_mlp_forward(self.h, X, X.shape[0], Y))
+        return Y
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        _mcheck(self.L, self.L.cc_mlp_grad(self.h, X, T, X.shape[0], C.byref(loss), g))
+        return loss.value, g
+
+    def train(self, X, T, epochs=10, batch=4096, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.zeros(epochs)
+        _mcheck(self.L, self.L.cc_mlp_train(self.h, X, T, X.shape[0], epochs, batch, lr, seed,
+                                            hist))
+        return hist
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cc_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

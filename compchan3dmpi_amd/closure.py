@@ -1,0 +1,203 @@
+"""Learned subgrid closure: training data from the GPU DNS, training with the
+library's own BF16-MFMA kernels, and the coarse run it drives.
+
+    python -m compchan3dmpi_amd.closure [--quick]
+
+prints one JSON line comparing the coarse run (nP = [64,32,48], the
+64x48x32 grid) with and without the closure against the DNS it was trained
+on (nP = [256,128,192] unless --quick): time-averaged wall shear (skin
+friction), mean-velocity profile, and GPU time per unit of simulated time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import formats
+from .channel import Channel, Mlp
+
+NIN, NOUT = 11, 9
+
+
+def normalise(X, T):
+    """Per-component mean/std of features and targets; returns the
+    normalised copies and the 40-float vector the device kernels take."""
+    fm = X[:, :NIN].mean(axis=0)
+    fs = X[:, :NIN].std(axis=0) + 1e-12
+    tm = T[:, :NOUT].mean(axis=0)
+    ts = T[:, :NOUT].std(axis=0) + 1e-12
+    Xn = np.zeros_like(X)
+    Tn = np.zeros_like(T)
+    Xn[:, :NIN] = (X[:, :NIN] - fm) / fs
+    Tn[:, :NOUT] = (T[:, :NOUT] - tm) / ts
+    norm = np.concatenate([fm, fs, tm, ts]).astype(np.float32)
+    return Xn.astype(np.float32), Tn.astype(np.float32), norm
+
+
+def mean_profile(var):
+    """Plane-mean streamwise velocity over the unique interior nodes."""
+    return var[1, :, 1:-1, 1:-1].mean(axis=(1, 2))
+
+
+def to_coarse(var, coarse, Mac=1.5):
+    """A fine state (7, n3, n2, n1) interpolated onto the coarse grid's
+    nodes: periodic-linear in x and y (the two periods are mapped onto each
+    other), linear in the wall-normal coordinate; u, v, w, rho, T are
+    interpolated, E and p recomputed, the periodic duplicates refilled."""
+    _, n3, n2, n1 = var.shape
+    c1, c2, c3 = coarse
+    _, _, x3f = formats.mesh(n1, n2, n3)
+    _, _, x3c = formats.mesh(c1, c2, c3)
+    q = np.stack([var[0], var[1], var[2], var[3], var[5]])[:, :, 1:-1, 1:-1]  # unique x, y
+
+    def periodic(a, axis, nc):
+        Pf = a.shape[axis]
+        f = np.arange(nc) * Pf / nc
+        i0 = np.floor(f).astype(int)
+        w = f - i0
+        a0 = np.take(a, i0 % Pf, axis=axis)
+        a1 = np.take(a, (i0 + 1) % Pf, axis=axis)
+        sh = [1] * a.ndim
+        sh[axis] = nc
+        return a0 * (1 - w.reshape(sh)) + a1 * w.reshape(sh)
+
+    q = periodic(q, 3, c1 - 2)
+    q = periodic(q, 2, c2 - 2)
+    out = np.empty((5, c3) + q.shape[2:])
+    for f in range(5):
+        for j in range(q.shape[2]):
+            for i in range(q.shape[3]):
+                out[f, :, j, i] = np.interp(x3c, x3f, q[f, :, j, i])
+    ggM = formats.GAMA * (formats.GAMA - 1) * Mac ** 2
+    v = np.zeros((7, c3, c2, c1))
+    core = (slice(None), slice(1, -1), slice(1, -1))
+    v[0][core] = out[0]
+    v[1][core] = out[1]
+    v[2][core] = out[2]
+    v[3][core] = out[3]
+    v[5][core] = out[4]
+    v[1:4, 0] = 0.0
+    v[1:4, -1] = 0.0
+    v[5][0] = 1.0
+    v[5][-1] = 1.0
+    v[4] = v[5] + ggM * 0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2)
+    v[6] = v[0] * v[5] / (formats.GAMA * Mac ** 2)
+    v[:, :, :, 0] = v[:, :, :, c1 - 2]
+    v[:, :, :, c1 - 1] = v[:, :, :, 1]
+    v[:, :, 0, :] = v[:, :, c2 - 2, :]
+    v[:, :, c2 - 1, :] = v[:, :, 1, :]
+    return v
+
+
+def run_dns(nP, dt, warm, nsnap, every, coarse, precision="mixed", istat=10, log=print):
+    """DNS from the seeded start: warm-up, then nsnap filtered snapshots.
+    Returns training samples, the wall shear and mean profile over the
+    window, the coarse initial state (the warmed-up DNS on the coarse
+    grid) and the GPU seconds per unit of simulated time."""
+    n1, n2, n3 = nP
+    ch = Channel(n1, n2, n3, dt=dt, istat=istat, precision=precision)
+    ch.upload(formats.laminar_ic(n1, n2, n3))
+    t0 = time.perf_counter()
+    if ch.step(warm):
+        raise RuntimeError("DNS blew up during warm-up")
+    t_warm = time.perf_counter() - t0
+    v0 = to_coarse(ch.download()[0], coarse)
+    Xs, Ts, taus, profs = [], [], [], []
+    t_win = 0.0
+    for s in range(nsnap):
+        t1 = time.perf_counter()
+        if ch.step(every):
+            raise RuntimeError("DNS blew up")
+        t_win += time.perf_counter() - t1
+        X, T = ch.filter_samples()
+        Xs.append(X)
+        Ts.append(T)
+        taus.append(ch.stats()[0].copy())
+        profs.append(mean_profile(ch.download()[0]))
+        log(f"  DNS snapshot {s + 1}/{nsnap}: {X.shape[0]} samples, tau = {taus[-1]}")
+    cost = (t_warm + t_win) / ((warm + nsnap * every) * dt)
+    ch.close()
+    return (np.concatenate(Xs), np.concatenate(Ts), np.array(taus), np.mean(profs, axis=0),
+            v0, cost)
+
+
+def run_coarse(nP, v0, dt, nsnap, every, mlp=None, norm=None, precision="mixed", istat=10):
+    n1, n2, n3 = nP
+    ch = Channel(n1, n2, n3, dt=dt, istat=istat, precision=precision)
+    ch.upload(v0)
+    if mlp is not None:
+        ch.set_closure(mlp, norm)
+    taus, profs = [], []
+    wall = 0.0
+    for s in range(nsnap):
+        t0 = time.perf_counter()
+        rc = ch.step(every)
+        wall += time.perf_counter() - t0
+        if rc:
+            raise RuntimeError("coarse run blew up")
+        taus.append(ch.stats()[0].copy())
+        profs.append(mean_profile(ch.download()[0]))
+    ch.close()
+    return np.array(taus), np.array(profs), wall / (nsnap * every * dt)
+
+
+def experiment(quick=False, log=print):
+    """DNS (training data and reference) -> training -> coarse runs started
+    from the filtered DNS, with and without the closure, over the same
+    window of simulated time."""
+    fine = (64, 32, 48) if quick else (256, 128, 192)
+    coarse = (18, 10, 12) if quick else (64, 32, 48)
+    dt_f = 1.5e-3 if quick else 9e-5
+    dt_c = 2e-3 if quick else 1.5e-3
+    T_warm, T_win = (0.3, 0.3) if quick else (2.7, 2.7)
+    nsnap = 4 if quick else 6
+    warm = int(round(T_warm / dt_f))
+    every = int(round(T_win / nsnap / dt_f))
+    log(f"DNS {fine}: {warm} warm-up steps, {nsnap} snapshots every {every} steps")
+    X, T, tau_dns, prof_dns, v0, cost_dns = run_dns(fine, dt_f, warm, nsnap, every, coarse, log=log)
+    Xn, Tn, norm = normalise(X, T)
+    mlp = Mlp(seed=1)
+    hist = mlp.train(Xn, Tn, epochs=6 if quick else 20, batch=2048 if quick else 8192,
+                     lr=2e-3, seed=2)
+    log(f"training loss per epoch: {np.array2string(hist, precision=4)}")
+    ce = int(round(T_win / nsnap / dt_c))
+    res = {}
+    for name, (m, nv) in (("no_closure", (None, None)), ("closure", (mlp, norm))):
+        taus, profs, cost = run_coarse(coarse, v0, dt_c, nsnap, ce, m, nv)
+        res[name] = {"tau_wall": taus.mean(axis=0), "cost": cost, "profile": profs.mean(axis=0)}
+    td = tau_dns.mean(axis=0)
+    cf_dns = 0.5 * (td[0] - td[1])
+    _, _, x3f = formats.mesh(*fine)
+    zc = np.array([x3f[4 * b:4 * b + 4].mean() for b in range(fine[2] // 4)])
+    _, _, x3c = formats.mesh(*coarse)
+    pf = np.array([prof_dns[4 * b:4 * b + 4].mean() for b in range(fine[2] // 4)])
+    out = {"dns_grid": list(fine), "coarse_grid": list(coarse), "samples": int(X.shape[0]),
+           "window": [T_warm, T_warm + T_win],
+           "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+           "tau_wall_dns": [float(td[0]), float(td[1])]}
+    for name, r in res.items():
+        tw = r["tau_wall"]
+        cf = 0.5 * (tw[0] - tw[1])
+        prof = np.interp(zc, x3c, r["profile"])
+        out[name] = {"tau_wall": [float(tw[0]), float(tw[1])],
+                     "skin_friction_rel_err": float(abs(cf - cf_dns) / abs(cf_dns)),
+                     "mean_u_profile_rel_err": float(np.abs(prof - pf).max() / np.abs(pf).max()),
+                     "cost_ratio_vs_dns": float(cost_dns / r["cost"])}
+    mlp.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--quick", action="store_true")
+    a = ap.parse_args()
+    out = experiment(a.quick, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -54,6 +54,8 @@ struct Chan {
   int blown_ijk[3] = {0, 0, 0};
   std::vector<double> x3;
   Inst inst{};      // descriptor of this instance for the fused kernels
+  void* mlp = nullptr;          // learned closure (cc_set_closure)
+  float *cnorm = nullptr, *cX = nullptr, *cY = nullptr;
   int variant = 0;  // tile shape of the stage kernel
   int kchunk = 0;   // planes per block (0 = automatic)
   // per-launch timing of the stage's main kernel (cc_profile)
@@ -137,6 +139,8 @@ void free_chan(Chan* c) {
   for (double** b : bufs)
     if (*b) (void)hipFree(*b);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  for (float* f : {c->cnorm, c->cX, c->cY})
+    if (f) (void)hipFree(f);
   if (c->halt) (void)hipFree(c->halt);
   if (c->blow) (void)hipFree(c->blow);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -349,8 +353,14 @@ int cc_step(void* handle, int nsteps) {
       return fail(CC_EINVAL, "fused precision modes are not built into this library");
 #else
       const int mixed = c->prm.precision == CC_MIXED;
-      launch_stage_fused(c->inst, mixed, c->variant, 1, step, c->s, ev(s, 0), ev(s, 1));
-      launch_stage_fused(c->inst, mixed, c->variant, 2, step, c->s, ev(s, 2), ev(s, 3));
+      for (int st = 1; st <= 2; st++) {
+        launch_stage_fused(c->inst, mixed, c->variant, st, step, c->s, ev(s, 2 * st - 2),
+                           ev(s, 2 * st - 1), c->mlp == nullptr);
+        if (c->mlp) {
+          closure_stage(c->mlp, c->inst, c->cnorm, c->cX, c->cY, st, c->s);
+          launch_wall_fast(c->inst, st, step, c->s);
+        }
+      }
 #endif
     }
     if (step % c->prm.istat == 0)
@@ -448,6 +458,49 @@ int cc_profile(void* handle, int mode, double* out) {
     out[0] = c->prof_ms;
     out[1] = (double)c->prof_n;
   }
+  return 0;
+}
+
+int cc_set_closure(void* handle, void* mlp, const float* norm) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  if (c->prm.precision == CC_FP64_STRICT) return fail(CC_EINVAL, "closure needs a fused mode");
+  if (mlp && !norm) return fail(CC_EINVAL, "closure needs its normalisation");
+  c->mlp = mlp;
+  if (!mlp) return 0;
+  const size_t NI = (size_t)c->g.plane * (c->n3 - 2);
+  if (!c->cnorm) HIPCHK(hipMalloc((void**)&c->cnorm, 40 * sizeof(float)));
+  if (!c->cX) HIPCHK(hipMalloc((void**)&c->cX, NI * 16 * sizeof(float)));
+  if (!c->cY) HIPCHK(hipMalloc((void**)&c->cY, NI * 16 * sizeof(float)));
+  HIPCHK(hipMemcpy(c->cnorm, norm, 40 * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int cc_filter_samples(void* handle, float* X, float* T, int* M) {
+  Chan* c = (Chan*)handle;
+  if (!c || !M) return fail(CC_EINVAL, "null argument");
+  if (c->prm.precision == CC_FP64_STRICT) return fail(CC_EINVAL, "filtering needs a fused mode");
+  const int bx = c->g.P1 / 4, by = c->g.P2 / 4, bz = c->n3 / 4;
+  if (bx < 3 || by < 3 || bz < 3) return fail(CC_EINVAL, "grid too small for a 4x4x4 filter");
+  const int ns = (bx - 2) * (by - 2) * (bz - 2);
+  *M = ns;
+  if (!X || !T) return 0;
+  std::vector<double> zc(bz);
+  for (int b = 0; b < bz; b++)
+    zc[b] = 0.25 * (c->x3[4 * b] + c->x3[4 * b + 1] + c->x3[4 * b + 2] + c->x3[4 * b + 3]);
+  double *dS = nullptr, *dz = nullptr;
+  float *dX = nullptr, *dT = nullptr;
+  HIPCHK(hipMalloc((void**)&dS, (size_t)bx * by * bz * 14 * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dz, bz * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dX, (size_t)ns * 16 * sizeof(float)));
+  HIPCHK(hipMalloc((void**)&dT, (size_t)ns * 16 * sizeof(float)));
+  HIPCHK(hipMemcpyAsync(dz, zc.data(), bz * sizeof(double), hipMemcpyHostToDevice, c->s));
+  box_filter(c->inst, bx, by, bz, dS, dz, dX, dT, c->s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(X, dX, (size_t)ns * 16 * sizeof(float), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(T, dT, (size_t)ns * 16 * sizeof(float), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (void* p : {(void*)dS, (void*)dz, (void*)dX, (void*)dT}) (void)hipFree(p);
   return 0;
 }
 

@@ -735,8 +735,14 @@ static void launch_one(const Inst& a, int stage, int step, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_fused<T, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
 }
 
+void launch_wall_fast(const Inst& a, int stage, int step, hipStream_t s) {
+  dim3 blk(64, 4);
+  dim3 grd((a.g.P1 + 63) / 64, (a.g.P2 + 3) / 4, 2);
+  hipLaunchKernelGGL(k_wall_fast, grd, blk, 0, s, a, stage, step);
+}
+
 void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int step,
-                        hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                        hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, bool wall) {
   if (ev0) (void)hipEventRecord(ev0, s);
   if (!mixed) {
     launch_one<double, 32, 16>(a, stage, step, s);
@@ -749,9 +755,7 @@ void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int st
     }
   }
   if (ev1) (void)hipEventRecord(ev1, s);
-  dim3 blk(64, 4);
-  dim3 grd((a.g.P1 + 63) / 64, (a.g.P2 + 3) / 4, 2);
-  hipLaunchKernelGGL(k_wall_fast, grd, blk, 0, s, a, stage, step);
+  if (wall) launch_wall_fast(a, stage, step, s);
 }
 
 template <typename T, int TX, int TY>

@@ -53,7 +53,21 @@ struct Inst {
 void tile_of(int variant, int& tx, int& ty);
 void set_tiling(Inst& a, int variant, int kchunk);
 void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int step,
-                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                        bool wall = true);
+// wall values and wall pressure after a stage (part of launch_stage_fused)
+void launch_wall_fast(const Inst& a, int stage, int step, hipStream_t s);
+
+// ---- cc_mlp.hip: learned subgrid closure ------------------------------------
+// Subtract the closure's subgrid-flux divergence from a stage's output; X, Y
+// hold 16 floats per interior node; norm: feature mean[11], std[11], target
+// mean[9], std[9].
+void closure_stage(void* mlp, const Inst& a, const float* norm, float* X, float* Y, int stage,
+                   hipStream_t s);
+// 4x4x4 box filter of an instance's state: S [bz][by][bx][14] sums, zc[bz]
+// block-centre heights; X, T receive (bx-2)(by-2)(bz-2) feature/target rows.
+void box_filter(const Inst& a, int bx, int by, int bz, double* S, const double* zc, float* X,
+                float* T, hipStream_t s);
 int batched_blocks_per_cu(int mixed, int variant);
 // d_work: int4 (instance, tile_x, tile_y, z-chunk) items
 void launch_stage_batched(const Inst* d_insts, const int4* d_work, int nwork, unsigned* counter,

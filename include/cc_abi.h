@@ -120,6 +120,37 @@ int cc_batch_step(void* batch, int nsteps);
  * out[0] = best ms per step, out[1..2] = tile (x, y), out[3] = z-chunk. */
 int cc_batch_tune(void* batch, int trial_steps, double* out);
 void cc_batch_destroy(void* batch);
+
+/* ---- learned subgrid closure (no counterpart in the reference, which has no
+ * LES) ----------------------------------------------------------------------
+ * An 11-64-64-64-9 GELU MLP: inputs the filtered velocity-gradient tensor
+ * (9, row-major du_i/dx_j), filtered density and temperature; outputs the
+ * subgrid stress (tau_11, tau_12, tau_13, tau_22, tau_23, tau_33) and heat
+ * flux (q_1, q_2, q_3).  Forward, backward and weight-gradient passes run on
+ * BF16 MFMA with FP32 accumulation; Adam keeps FP32 master weights.  Feature
+ * and target rows are 16 floats (zero-padded).  Parameters are flattened as
+ * W1[16][64], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], W4[64][16],
+ * b4[16]. */
+int cc_mlp_nparams(void);
+int cc_mlp_create(unsigned seed, void** mlp);
+int cc_mlp_set_params(void* mlp, const float* params);
+int cc_mlp_get_params(void* mlp, float* params);
+int cc_mlp_forward(void* mlp, const float* X, int N, float* Y);
+/* loss = mean over N samples and 9 outputs of (Y - T)^2, and its gradient. */
+int cc_mlp_grad(void* mlp, const float* X, const float* T, int N, double* loss, float* grad);
+/* Adam over shuffled mini-batches; loss_hist[epochs] (may be NULL). */
+int cc_mlp_train(void* mlp, const float* X, const float* T, int N, int epochs, int batch,
+                 float lr, unsigned seed, double* loss_hist);
+void cc_mlp_destroy(void* mlp);
+const char* cc_mlp_last_error(void);
+/* Training samples from an instance's current state: 4x4x4 box filter
+ * (Favre averages), features and subgrid targets at the interior cells of
+ * the coarse lattice.  With X or T NULL only *M (the sample count) is set. */
+int cc_filter_samples(void* handle, float* X, float* T, int* M);
+/* Add the closure's subgrid fluxes (momentum tau_ij, energy q_j + ggM u_i
+ * tau_ij) to every stage of cc_step; norm[40] = feature mean[11], std[11],
+ * target mean[9], std[9].  mlp NULL removes it. */
+int cc_set_closure(void* handle, void* mlp, const float* norm);
 const char* cc_last_error(void);
 int cc_abi_version(void);
 

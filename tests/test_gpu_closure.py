@@ -1,0 +1,140 @@
+"""The learned closure's device kernels against the FP64 NumPy reference
+(oracle/mlp_ref.py): BF16-MFMA forward and gradients, training, the 4x4x4
+box filter, and the coupling to a coarse run."""
+import numpy as np
+import pytest
+
+from compchan3dmpi_amd import closure, formats
+from compchan3dmpi_amd.channel import Channel, Mlp
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def rand_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, 16), dtype=np.float32)
+    X[:, :11] = rng.normal(size=(n, 11))
+    T = np.zeros((n, 16), dtype=np.float32)
+    T[:, :9] = rng.normal(size=(n, 9))
+    return X, T
+
+
+def test_forward_matches_reference():
+    banner("MLP forward (BF16 MFMA) vs FP64 reference")
+    m = Mlp(seed=5)
+    P = m.params.astype(np.float64)
+    X, _ = rand_problem(1000, 1)          # not a multiple of 16 or 64
+    Y = m.forward(X)
+    Yr, _ = mlp_ref.forward(P, X, emulate_bf16=True)
+    Y64, _ = mlp_ref.forward(P, X, emulate_bf16=False)
+    err = np.abs(Y[:, :9] - Yr[:, :9]).max() / np.abs(Yr[:, :9]).max()
+    err64 = np.abs(Y[:, :9] - Y64[:, :9]).max() / np.abs(Y64[:, :9]).max()
+    print(f"  max rel err vs bf16-emulating ref {err:.2e}, vs FP64 ref {err64:.2e}", flush=True)
+    assert err < 1e-2 and err64 < 3e-2
+    assert np.all(Y[:, 9:] == 0)
+
+
+def test_gradient_matches_reference():
+    banner("MLP loss gradient (BF16 MFMA) vs FP64 reference")
+    m = Mlp(seed=6)
+    P = m.params.astype(np.float64)
+    X, T = rand_problem(777, 2)
+    loss, g = m.grad(X, T)
+    lr, gr = mlp_ref.loss_grad(P, X, T, emulate_bf16=True)
+    _, g64 = mlp_ref.loss_grad(P, X, T, emulate_bf16=False)
+    assert abs(loss - lr) <= 1e-2 * lr
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    cos = g @ g64 / (np.linalg.norm(g) * np.linalg.norm(g64))
+    print(f"  grad rel err vs bf16 ref {rel:.2e}, cosine vs FP64 ref {cos:.6f}", flush=True)
+    assert rel < 2e-2 and cos > 0.999
+    for blk, (o, *shape) in mlp_ref.OFF.items():
+        n = int(np.prod(shape))
+        a, b = g[o:o + n], gr[o:o + n]
+        assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-12, blk
+
+
+def test_training_fits_a_teacher_network():
+    banner("training on the device: fit a fixed teacher network")
+    teacher = Mlp(seed=11)
+    X, _ = rand_problem(20000, 3)
+    T = teacher.forward(X)
+    student = Mlp(seed=12)
+    l0, _ = student.grad(X, T)
+    hist = student.train(X, T, epochs=15, batch=1024, lr=3e-3, seed=4)
+    l1, _ = student.grad(X, T)
+    print(f"  loss {l0:.4e} -> {l1:.4e}; per epoch {np.array2string(hist, precision=3)}", flush=True)
+    assert l1 < 0.1 * l0
+
+
+def box_reference(var, ggM):
+    """NumPy 4x4x4 Favre box filter of a downloaded state (unique nodes)."""
+    v = var[:, :, 1:-1, 1:-1]                    # (7, n3, P2, P1)
+    r, u, vv, w, T = v[0], v[1], v[2], v[3], v[5]
+    n3, P2, P1 = r.shape
+    bx, by, bz = P1 // 4, P2 // 4, n3 // 4
+
+    def box(a):
+        a = a[:4 * bz, :4 * by, :4 * bx]
+        return a.reshape(bz, 4, by, 4, bx, 4).mean(axis=(1, 3, 5))
+    R = box(r)
+    U = [box(r * x) / R for x in (u, vv, w)]
+    Tt = box(r * T) / R
+    tau = [box(r * a * b) - R * A * B for (a, A), (b, B) in
+           [((u, U[0]), (u, U[0])), ((u, U[0]), (vv, U[1])), ((u, U[0]), (w, U[2])),
+            ((vv, U[1]), (vv, U[1])), ((vv, U[1]), (w, U[2])), ((w, U[2]), (w, U[2]))]]
+    q = [box(r * a * T) - R * A * Tt for a, A in ((u, U[0]), (vv, U[1]), (w, U[2]))]
+    return R, U, Tt, tau, q
+
+
+def test_box_filter_matches_numpy():
+    banner("4x4x4 box filter vs NumPy")
+    n1, n2, n3 = 34, 18, 16
+    ch = Channel(n1, n2, n3, dt=1e-3, istat=5, precision="fp64")
+    ch.upload(formats.laminar_ic(n1, n2, n3))
+    ch.step(3)
+    X, T = ch.filter_samples()
+    var, _, _ = ch.download()
+    R, U, Tt, tau, q = box_reference(var, 1.26)
+    sl = (slice(1, -1), slice(1, -1), slice(1, -1))
+    assert np.allclose(X[:, 9], R[sl].ravel(), rtol=1e-6)
+    assert np.allclose(X[:, 10], Tt[sl].ravel(), rtol=1e-6)
+    for c in range(6):
+        assert np.allclose(T[:, c], tau[c][sl].ravel(), rtol=1e-4, atol=1e-9), c
+    for c in range(3):
+        assert np.allclose(T[:, 6 + c], q[c][sl].ravel(), rtol=1e-4, atol=1e-9), c
+    x1, x2, x3 = formats.mesh(n1, n2, n3)
+    hx = 4 * (x1[1] - x1[0])
+    dudx = (U[0][1:-1, 1:-1, 2:] - U[0][1:-1, 1:-1, :-2]) / (2 * hx)
+    assert np.allclose(X[:, 0], dudx.ravel(), rtol=1e-4, atol=1e-8)
+    ch.close()
+
+
+def test_zero_closure_leaves_the_step_unchanged():
+    banner("closure with a zero network is the plain step")
+    n1, n2, n3 = 18, 10, 12
+    v0 = formats.laminar_ic(n1, n2, n3)
+    a = Channel(n1, n2, n3, dt=2e-3, istat=5, precision="mixed")
+    a.upload(v0)
+    a.step(10)
+    m = Mlp(seed=0)
+    m.params = np.zeros(m.nparams, dtype=np.float32)
+    norm = np.concatenate([np.zeros(11), np.ones(11), np.zeros(9), np.ones(9)]).astype(np.float32)
+    b = Channel(n1, n2, n3, dt=2e-3, istat=5, precision="mixed")
+    b.upload(v0)
+    b.set_closure(m, norm)
+    b.step(10)
+    assert np.array_equal(a.download()[0], b.download()[0])
+
+
+def test_quick_closure_experiment():
+    banner("closure experiment (quick sizes)")
+    out = closure.experiment(quick=True, log=lambda *x: print(" ", *x, flush=True))
+    print(" ", out, flush=True)
+    assert out["train_loss_first_last"][1] < out["train_loss_first_last"][0]
+    for k in ("no_closure", "closure"):
+        assert np.isfinite(out[k]["skin_friction_rel_err"])
