@@ -29,7 +29,14 @@ ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_d
                "cc_batch_create", "cc_batch_member", "cc_batch_step", "cc_batch_tune",
                "cc_batch_destroy", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
                "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
-               "cc_mlp_destroy", "cc_mlp_last_error", "cc_filter_samples", "cc_set_closure")
+               "cc_mlp_destroy", "cc_mlp_last_error", "cc_filter_samples", "cc_set_closure",
+               "cc_set_wall", "cc_wall_preset")
+
+# Study names accepted by cc_wall_preset: the reference's wall-condition
+# parameter studies (v1.9_STWNV, v1.9_RD, v1.9_groov, v1.9_groov_strat,
+# v1.9_strat).
+STUDIES = ("s1", "s2", "s3", "s4", "s5", "s6", "rd1", "rd2", "rd3", "g1", "gst1", "gst2",
+           "gst3", "gst4", "gst5", "st1", "st2", "st3", "st4")
 
 
 class cc_params(C.Structure):
@@ -38,6 +45,17 @@ class cc_params(C.Structure):
                 ("Re", C.c_double), ("Mac", C.c_double), ("dt", C.c_double),
                 ("PexVal", C.c_double), ("PeyVal", C.c_double), ("PezVal", C.c_double),
                 ("Ri_b", C.c_double), ("slip_len", C.c_double)]
+
+
+class cc_wall(C.Structure):
+    """Wall condition (include/cc_abi.h cc_wall); zero = the reference wall."""
+    _fields_ = [("vel", C.c_int), ("stripes", C.c_int), ("temp", C.c_int), ("reserved", C.c_int),
+                ("amp", C.c_double), ("kx", C.c_double), ("omega", C.c_double),
+                ("phase_odd", C.c_double), ("disc_omega", C.c_double), ("grad", C.c_double),
+                ("Tw", C.c_double), ("dTdz", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class CCError(RuntimeError):
@@ -92,12 +110,14 @@ def load_library():
     L.cc_mlp_last_error.restype = C.c_char_p
     L.cc_filter_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
     L.cc_set_closure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.cc_set_wall.argtypes = [C.c_void_p, C.c_int, C.POINTER(cc_wall), C.c_void_p, C.c_void_p]
+    L.cc_wall_preset.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(cc_wall)]
     L.cc_last_error.restype = C.c_char_p
     for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
               "cc_blown", "cc_profile", "cc_abi_version", "cc_batch_create", "cc_batch_step",
               "cc_batch_tune", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
               "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
-              "cc_filter_samples", "cc_set_closure"):
+              "cc_filter_samples", "cc_set_closure", "cc_set_wall", "cc_wall_preset"):
         getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -107,6 +127,15 @@ def _check(L, rc):
     if rc < 0:
         raise CCError(f"cc error {rc}: {L.cc_last_error().decode()}")
     return rc
+
+
+def wall_preset(name):
+    """(wall, cc_wall) of a named study (host-only call, no device needed)."""
+    L = load_library()
+    w = cc_wall()
+    wall = C.c_int()
+    _check(L, L.cc_wall_preset(name.encode(), C.byref(wall), C.byref(w)))
+    return wall.value, w
 
 
 def _params(n1, n2, n3, Re=3000.0, Mac=1.5, dt=1e-4, istat=10, PexVal=2.0, PeyVal=2.0,
@@ -205,6 +234,32 @@ class Channel:
         nv = np.ascontiguousarray(norm, dtype=np.float32)
         _check(self.L, self.L.cc_set_closure(self.h, mlp.h, nv.ctypes.data_as(C.c_void_p)))
         self._closure = (mlp, nv)  # keep both alive while attached
+
+    def set_wall(self, wall, spec=None, x1=None, x2=None):
+        """Wall condition of wall 0 (bottom) or 1 (top): spec is a cc_wall, a
+        dict of its fields, a study name (applied to the study's wall; then
+        `wall` may be None) or None for the reference wall.  x1 (n1), x2 (n2):
+        the solution file's coordinates (None: the analytic mesh)."""
+        if isinstance(spec, str):
+            w_study, spec = wall_preset(spec)
+            wall = w_study if wall is None else wall
+        elif isinstance(spec, dict):
+            spec = cc_wall(**spec)
+        elif spec is None:
+            spec = cc_wall()
+        keep = []
+
+        def ptr(a, n):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != (n,):
+                raise ValueError(f"coordinate array has shape {a.shape}, expected ({n},)")
+            keep.append(a)
+            return a.ctypes.data
+
+        _check(self.L, self.L.cc_set_wall(self.h, int(wall), C.byref(spec), ptr(x1, self.n[0]),
+                                          ptr(x2, self.n[1])))
 
     def blown(self):
         ijk = (C.c_int * 3)()

@@ -42,7 +42,12 @@ struct Chan {
   int n1 = 0, n2 = 0, n3 = 0;
   hipStream_t s = nullptr;
   double *UA = nullptr, *UB = nullptr, *W = nullptr, *rw = nullptr, *pw = nullptr;
-  double *uwall = nullptr, *vwall = nullptr;  // wall slip velocities [2][plane]
+  double *uwall = nullptr, *vwall = nullptr;  // wall velocities [2][plane]
+  double *twall = nullptr, *mwall = nullptr;  // wall T and mu = T^0.7 [2][plane]
+  double *ring = nullptr;                     // duplicate wall nodes
+  double *dwall = nullptr;                    // their divergence correction
+  double *x1f = nullptr, *x2f = nullptr;      // file coordinates for the studies
+  bool in_batch = false;
   double *jac = nullptr, *invJac = nullptr, *force = nullptr, *consts = nullptr;
   double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr, *dbuf = nullptr;
   int* halt = nullptr;
@@ -67,7 +72,7 @@ struct Chan {
   WallP wallp() const {
     if (prm.precision == CC_FP64_STRICT)
       return WallP{W + 5 * g.field, (long long)(n3 - 1) * g.plane, nullptr, nullptr};
-    return WallP{pw, g.plane, uwall, vwall};
+    return WallP{pw, g.plane, uwall, vwall, twall, mwall, ring, inst.wflags};
   }
 };
 
@@ -127,13 +132,21 @@ void setup_mesh(Chan& c) {
   ph.gkRP = gama * ph.invRe * kInvPr;
   ph.Ri = c.prm.Ri_b;
   ph.Ls = c.prm.slip_len;
+  // x1, x2 of the analytic mesh (comdata.f90:81-100) until cc_set_wall
+  // brings the file's values
+  std::vector<double> x1(n1, 0.0), x2(n2, 0.0);
+  for (int i = 0; i < n1 - 1; i++) x1[i + 1] = x1[i] + ph.dx;
+  for (int j = 0; j < n2 - 1; j++) x2[j + 1] = x2[j] + ph.dy;
+  (void)hipMemcpy(c.x1f, x1.data(), n1 * sizeof(double), hipMemcpyHostToDevice);
+  (void)hipMemcpy(c.x2f, x2.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
   (void)hipMemcpy(c.jac, jac.data(), n3 * sizeof(double), hipMemcpyHostToDevice);
   (void)hipMemcpy(c.invJac, invJac.data(), n3 * sizeof(double), hipMemcpyHostToDevice);
 }
 
 void free_chan(Chan* c) {
   if (!c) return;
-  double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->uwall, &c->vwall, &c->jac,
+  double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->uwall, &c->vwall, &c->twall,
+                     &c->mwall, &c->ring, &c->dwall, &c->x1f, &c->x2f, &c->jac,
                      &c->invJac, &c->force,
                      &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red, &c->dbuf};
   for (double** b : bufs)
@@ -255,7 +268,11 @@ int cc_create(const cc_params* p, void** handle) {
   if (hipStreamCreate(&c->s) != hipSuccess || al(&c->UA, 5 * F) != hipSuccess ||
       al(&c->UB, 5 * F) != hipSuccess || al(&c->rw, 2 * PL) != hipSuccess ||
       al(&c->pw, 2 * PL) != hipSuccess || al(&c->uwall, 2 * PL) != hipSuccess ||
-      al(&c->vwall, 2 * PL) != hipSuccess || al(&c->jac, n3) != hipSuccess ||
+      al(&c->vwall, 2 * PL) != hipSuccess || al(&c->twall, 2 * PL) != hipSuccess ||
+      al(&c->mwall, 2 * PL) != hipSuccess || al(&c->dwall, 2 * PL) != hipSuccess ||
+      al(&c->ring, 2 * kRingFields * (size_t)ring_size(c->g.P1, c->g.P2)) != hipSuccess ||
+      al(&c->x1f, p->n1) != hipSuccess || al(&c->x2f, p->n2) != hipSuccess ||
+      al(&c->jac, n3) != hipSuccess ||
       al(&c->invJac, n3) != hipSuccess || al(&c->force, 2 + n3) != hipSuccess ||
       al(&c->consts, kNStat * n3) != hipSuccess || al(&c->stats, kNStat * n3 + 2) != hipSuccess ||
       al(&c->Vfile, 7 * full) != hipSuccess || al(&c->red, 4096) != hipSuccess ||
@@ -276,6 +293,16 @@ int cc_create(const cc_params* p, void** handle) {
     a.rw = c->rw;
     a.uw = c->uwall;
     a.vw = c->vwall;
+    a.tw = c->twall;
+    a.mw = c->mwall;
+    a.ring = c->ring;
+    a.dw = c->dwall;
+    a.vfile = c->Vfile;
+    a.x1f = c->x1f;
+    a.x2f = c->x2f;
+    std::memset(a.ws, 0, sizeof a.ws);
+    a.wflags = 0;
+    a.time = 0.0;
     a.jac = c->jac;
     a.invJac = c->invJac;
     a.force = c->force;
@@ -300,15 +327,11 @@ int cc_upload(void* handle, const double* var, int nstep, double time) {
   const size_t full = (size_t)c->n1 * c->n2 * c->n3;
   HIPCHK(hipMemcpyAsync(c->Vfile, var, 7 * full * sizeof(double), hipMemcpyHostToDevice, c->s));
   launch_upload(c->g, c->ph, c->n1, c->n2, c->Vfile, c->UA, c->UB, c->W, c->wallp(), c->rw,
-                c->consts, c->s);
+                c->dwall, c->consts, c->s);
   HIPCHK(hipGetLastError());
   // forcing state: zero until the first statistics step (the reference's
   // static initialisation of tauxzL, tauxzU, stat1Spc)
   HIPCHK(hipMemsetAsync(c->force, 0, (2 + c->n3) * sizeof(double), c->s));
-  // walls start at rest (the file's wall velocity is zero for every
-  // reference state; the first wall update sets the slip velocity)
-  HIPCHK(hipMemsetAsync(c->uwall, 0, 2 * c->g.plane * sizeof(double), c->s));
-  HIPCHK(hipMemsetAsync(c->vwall, 0, 2 * c->g.plane * sizeof(double), c->s));
   HIPCHK(hipMemsetAsync(c->halt, 0, sizeof(int), c->s));
   const unsigned long long big = ~0ull;
   HIPCHK(hipMemcpyAsync(c->blow, &big, sizeof(big), hipMemcpyHostToDevice, c->s));
@@ -341,6 +364,7 @@ int cc_step(void* handle, int nsteps) {
   auto ev = [&](int s, int k) -> hipEvent_t { return c->prof ? c->evpool[(size_t)4 * s + k] : nullptr; };
   for (int s = 0; s < nsteps; s++) {
     const int step = c->nstep + 1;
+    c->inst.time = c->time + c->prm.dt;  // the reference advances time first
     if (c->prm.precision == CC_FP64_STRICT) {
       launch_stage_strict(c->g, c->ph, c->UA, c->UA, c->UB, c->W, c->rw, c->jac, c->invJac,
                           c->force, 1, c->halt, c->blow, step, c->n1, c->n2, c->s, ev(s, 0),
@@ -528,6 +552,91 @@ int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
 }
 
 
+int cc_set_wall(void* handle, int wall, const cc_wall* w, const double* x1, const double* x2) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  if (wall != 0 && wall != 1) return fail(CC_EINVAL, "wall must be 0 (bottom) or 1 (top)");
+  cc_wall z{};
+  const cc_wall& s = w ? *w : z;
+  if (s.vel < CC_WALL_NOSLIP || s.vel > CC_WALL_STRIPES || s.temp < CC_WALLT_ONE ||
+      s.temp > CC_WALLT_GRAD || s.stripes < CC_STRIPES_BANDS || s.stripes > CC_STRIPES_WAVY)
+    return fail(CC_EINVAL, "unknown wall condition");
+  const bool study = s.vel != CC_WALL_NOSLIP || s.temp != CC_WALLT_ONE;
+  if (study) {
+    if (c->prm.precision == CC_FP64_STRICT)
+      return fail(CC_EINVAL, "study wall conditions are not part of the reference-order path");
+    if (c->in_batch) return fail(CC_EINVAL, "study wall conditions are per single instance");
+    if (c->prm.slip_len != 0.0)
+      return fail(CC_EINVAL, "a study wall condition replaces the slip wall: slip_len must be 0");
+    if (s.temp == CC_WALLT_FIXED && !(s.Tw > 0.0))
+      return fail(CC_EINVAL, "wall temperature must be positive");
+  }
+  if (c->loaded && !c->fresh)
+    return fail(CC_ESTATE, "set wall conditions before the first step after an upload");
+  static_assert(sizeof(cc_wall) == sizeof(WallSpec), "cc_wall and WallSpec differ");
+  std::memcpy(&c->inst.ws[wall], &s, sizeof(WallSpec));
+  c->inst.wflags = (c->inst.wflags & ~(1 << wall)) | (study ? (1 << wall) : 0);
+  if (x1) HIPCHK(hipMemcpy(c->x1f, x1, c->n1 * sizeof(double), hipMemcpyHostToDevice));
+  if (x2) HIPCHK(hipMemcpy(c->x2f, x2, c->n2 * sizeof(double), hipMemcpyHostToDevice));
+  if (c->loaded) {  // the constant plane-sum terms depend on the wall conditions
+    launch_wall_init(c->g, c->ph, c->n1, c->n2, c->Vfile, c->wallp(), c->dwall, c->consts, c->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  return 0;
+}
+
+int cc_wall_preset(const char* name, int* wall, cc_wall* w) {
+  if (!name || !wall || !w) return fail(CC_EINVAL, "null argument");
+  cc_wall s{};
+  *wall = 0;
+  // the single-precision literal 3.14 of main_v1.9_s3.f90:610 and _s4.f90
+  const double pi_f = (double)3.14f;
+  const std::string n(name);
+  auto wave = [&](double amp, double kx, double om, double ph) {
+    s.vel = CC_WALL_WAVE;
+    s.amp = amp;
+    s.kx = kx;
+    s.omega = om;
+    s.phase_odd = ph;
+  };
+  auto stripes = [&](int pat, double grad, double Tw) {
+    s.vel = CC_WALL_STRIPES;
+    s.stripes = pat;
+    s.grad = grad;
+    s.temp = CC_WALLT_FIXED;
+    s.Tw = Tw;
+  };
+  if (n.empty() || n == "ref") {
+  } else if (n == "s1") wave(0.7, 10.0, 3.0, 0.0);       // main_v1.9_s1.f90:609
+  else if (n == "s2") wave(0.5, 10.0, 3.0, 0.0);         // main_v1.9_s2.f90:609
+  else if (n == "s3") wave(0.5, 10.0, -3.0, pi_f);       // main_v1.9_s3.f90:609-610
+  else if (n == "s4") wave(0.5, 3.75, 3.0, pi_f);        // main_v1.9_s4.f90:609-610
+  else if (n == "s5") wave(0.7, 3.75, -3.0, 0.0);        // main_v1.9_s5.f90:609
+  else if (n == "s6") wave(0.3, 10.0, 3.0, 0.0);         // main_v1.9_s6.f90:609
+  else if (n == "rd1" || n == "rd2" || n == "rd3") {     // main_v1.9_rd1.f90:611-640
+    s.vel = CC_WALL_DISCS;
+    s.disc_omega = n == "rd3" ? 2.0 : 3.0;
+  } else if (n == "g1") {                                // main_v1.9_g1.f90:609-618
+    s.vel = CC_WALL_STRIPES;
+    s.stripes = CC_STRIPES_BANDS;
+  } else if (n == "gst1") stripes(CC_STRIPES_BANDS, 0.0, 0.3);
+  else if (n == "gst2") stripes(CC_STRIPES_BANDS, 0.5, 0.3);
+  else if (n == "gst3" || n == "gst4") stripes(CC_STRIPES_DIAGONAL, 0.0, 0.3);
+  else if (n == "gst5") stripes(CC_STRIPES_WAVY, 0.0, 0.3);
+  else if (n == "st1" || n == "st2") {                   // main_v1.9_st1.f90:612
+    s.temp = CC_WALLT_FIXED;
+    s.Tw = n == "st1" ? 0.2 : 1.8;
+  } else if (n == "st3" || n == "st4") {                 // main_v1.9_st3.f90:612
+    s.temp = CC_WALLT_GRAD;
+    s.dTdz = n == "st3" ? 2.0 : 12.0;
+  } else {
+    return fail(CC_EINVAL, "unknown study: " + n);
+  }
+  *w = s;
+  return 0;
+}
+
 int cc_batch_create(const cc_params* p, int n, void** batch) {
   if (!p || !batch || n < 1) return fail(CC_EINVAL, "null argument or empty batch");
   *batch = nullptr;
@@ -545,6 +654,7 @@ int cc_batch_create(const cc_params* p, int n, void** batch) {
       return rc;
     }
     Chan* c = (Chan*)h;
+    c->in_batch = true;
     b->m.push_back(c);
     b->maxP1 = std::max(b->maxP1, c->g.P1);
     b->maxP2 = std::max(b->maxP2, c->g.P2);

@@ -48,18 +48,34 @@ __device__ __forceinline__ void stat_terms(const Prim& P, double q[kNStat]) {
 }
 
 // Wall state after the boundary update (main_v1.9.f90:624-639): u=v=w=0
-// (or the slip velocity), T=1, E=T+ggM/2|u|^2, rho = gama Ma^2 p / T.
+// (or the slip or study velocity), T=1 (or the study temperature),
+// E=T+ggM/2|u|^2, rho = gama Ma^2 p / T.
 __device__ __forceinline__ Prim wall_prim(const Phys& ph, double rho, double p, double u = 0.0,
-                                          double v = 0.0) {
+                                          double v = 0.0, double T = 1.0) {
   Prim P;
   P.rho = rho;
   P.u = u;
   P.v = v;
   P.w = 0.0;
-  P.T = 1.0;
-  P.E = 1.0 + ph.ggM * 0.5 * (u * u + v * v + 0.0 * 0.0);
+  P.T = T;
+  P.E = T + ph.ggM * 0.5 * (u * u + v * v + 0.0 * 0.0);
   P.p = p;
   return P;
+}
+
+// State of a wall-plane duplicate node from the ring (study walls).
+__device__ __forceinline__ Prim ring_prim(const Phys& ph, const double* rg, int NRG, int slot) {
+  return wall_prim(ph, rg[3 * NRG + slot], rg[4 * NRG + slot], rg[slot], rg[NRG + slot],
+                   rg[2 * NRG + slot]);
+}
+
+// Ring slot of the duplicate at Fortran 0-based (i, j) of a wall plane, or -1.
+__device__ __forceinline__ int ring_slot(int n1, int n2, int P1, int P2, int i, int j) {
+  const bool ei = (i == 0 || i == n1 - 1), ej = (j == 0 || j == n2 - 1);
+  if (ei && ej) return 2 * P2 + 2 * P1 + (i == 0 ? 0 : 1) + (j == 0 ? 0 : 2);
+  if (ei) return (i == 0 ? 0 : P2) + (j - 1);
+  if (ej) return 2 * P2 + (j == 0 ? 0 : P1) + (i - 1);
+  return -1;
 }
 
 // ---- upload: conserved state from var (main_v1.9.f90:284-294) -----------
@@ -107,7 +123,7 @@ __global__ void k_upload(Geom g, Phys ph, int n1, int n2, const double* __restri
 // (:645 loops over updated i,j only) and whose other values are set by the
 // wall update (:624-639).
 __global__ void k_consts(Geom g, Phys ph, int n1, int n2, const double* __restrict__ V,
-                         double* __restrict__ C) {
+                         double* __restrict__ C, int wflags) {
   const int kk = blockIdx.x * blockDim.x + threadIdx.x;
   const int n3 = g.n3;
   if (kk >= n3) return;
@@ -129,7 +145,9 @@ __global__ void k_consts(Geom g, Phys ph, int n1, int n2, const double* __restri
         stat_terms(P, tq);
         for (int q = 0; q < kNStat; q++) acc[q] += tq[q];
       }
-  } else if (ph.Ls == 0.0) {  // slip walls: the ring holds periodic images
+  } else if (ph.Ls == 0.0 && !((wflags >> (kk != 0)) & 1)) {
+    // (slip walls: the ring holds periodic images; study walls: the ring
+    // is updated every stage and summed by k_planestats)
     for (int j = 0; j < n2; j++)
       for (int i = 0; i < n1; i++) {
         if (!(i == 0 || i == n1 - 1 || j == 0 || j == n2 - 1)) continue;
@@ -140,6 +158,55 @@ __global__ void k_consts(Geom g, Phys ph, int n1, int n2, const double* __restri
       }
   }
   for (int q = 0; q < kNStat; q++) C[q * n3 + kk] = acc[q];
+}
+
+// Wall arrays of the fused path from the uploaded file: before the first
+// wall update the wall planes hold the file's values (main_v1.9.f90:272-294:
+// var as read, mu = T**0.7), duplicates included.  Threads [0, 2 plane) fill
+// the own wall nodes, the rest the duplicate ring (u, v, T, rho, p).
+__global__ void k_wall_init(Geom g, Phys ph, int n1, int n2, const double* __restrict__ V,
+                            WallP pw, double* __restrict__ dw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n3 = g.n3, NRG = ring_size(g.P1, g.P2);
+  if (t < 2 * g.plane) {
+    const int top = (int)(t / g.plane);
+    const long long n = t % g.plane;
+    const int ii = (int)(n % g.P1), jj = (int)(n / g.P1), k = top ? n3 - 1 : 0;
+    auto at = [&](int i, int j, int c) { return V[fidx(n1, n2, n3, i, j, k, c)]; };
+    pw.uw[t] = at(ii + 1, jj + 1, 1);
+    pw.vw[t] = at(ii + 1, jj + 1, 2);
+    const double T = at(ii + 1, jj + 1, 5);
+    pw.tw[t] = T;
+    pw.mw[t] = pow(T, 0.7);
+    // divergence correction of the edge nodes: file duplicate vs the
+    // periodic image the stage kernel reads (see wall_node)
+    double dc = 0.0;
+    if (ii == 0) dc += (at(n1 - 2, jj + 1, 1) - at(0, jj + 1, 1)) * 0.5 * ph.invdx;
+    if (ii == g.P1 - 1) dc += (at(n1 - 1, jj + 1, 1) - at(1, jj + 1, 1)) * 0.5 * ph.invdx;
+    if (jj == 0) dc += (at(ii + 1, n2 - 2, 2) - at(ii + 1, 0, 2)) * 0.5 * ph.invdy;
+    if (jj == g.P2 - 1) dc += (at(ii + 1, n2 - 1, 2) - at(ii + 1, 1, 2)) * 0.5 * ph.invdy;
+    dw[t] = (2.0 / 3.0) * pow(T, 0.7) * ph.invRe * dc;  // as in wall_node
+    return;
+  }
+  const long long q = t - 2 * g.plane;
+  if (q >= 2LL * NRG) return;
+  const int top = (int)(q / NRG), slot = (int)(q % NRG), k = top ? n3 - 1 : 0;
+  int i, j;  // Fortran 0-based position of the slot
+  if (slot < g.P2) { i = 0; j = slot + 1; }
+  else if (slot < 2 * g.P2) { i = n1 - 1; j = slot - g.P2 + 1; }
+  else if (slot < 2 * g.P2 + g.P1) { j = 0; i = slot - 2 * g.P2 + 1; }
+  else if (slot < 2 * g.P2 + 2 * g.P1) { j = n2 - 1; i = slot - 2 * g.P2 - g.P1 + 1; }
+  else {
+    const int c = slot - 2 * g.P2 - 2 * g.P1;
+    i = (c & 1) ? n1 - 1 : 0;
+    j = (c & 2) ? n2 - 1 : 0;
+  }
+  double* rg = pw.ring + (long long)top * kRingFields * NRG;
+  rg[slot] = V[fidx(n1, n2, n3, i, j, k, 1)];
+  rg[NRG + slot] = V[fidx(n1, n2, n3, i, j, k, 2)];
+  rg[2 * NRG + slot] = V[fidx(n1, n2, n3, i, j, k, 5)];
+  rg[3 * NRG + slot] = V[fidx(n1, n2, n3, i, j, k, 0)];
+  rg[4 * NRG + slot] = V[fidx(n1, n2, n3, i, j, k, 6)];
 }
 
 // ---- download: the reference's var(n1,n2,n3,7) with every halo value ----
@@ -160,7 +227,12 @@ __global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __rest
     for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, n3, i, j, k, c)] = V[fidx(n1, n2, n3, i, j, k, c)];
     return;
   }
-  if (wall && (ei || ej) && ph.Ls == 0.0) {
+  const int top = k == n3 - 1;
+  if (wall && (ei || ej) && ((pw.wflags >> top) & 1)) {
+    const int NRG = ring_size(g.P1, g.P2);
+    P = ring_prim(ph, pw.ring + (long long)top * kRingFields * NRG, NRG,
+                  ring_slot(n1, n2, g.P1, g.P2, i, j));
+  } else if (wall && (ei || ej) && ph.Ls == 0.0) {
     const double p = V[fidx(n1, n2, n3, i, j, k, 6)];
     P = wall_prim(ph, ph.gM2 * p / 1.0, p);
   } else {
@@ -168,10 +240,9 @@ __global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __rest
     const int jj = (j == 0) ? g.P2 - 1 : (j == n2 - 1 ? 0 : j - 1);
     const long long n2d = (long long)jj * g.P1 + ii;
     if (wall) {
-      const bool top = (k == n3 - 1);
       const long long wo = (top ? g.plane : 0) + n2d;
       P = wall_prim(ph, rw[wo], pw.p[(top ? pw.top_off : 0) + n2d], pw.uw ? pw.uw[wo] : 0.0,
-                    pw.vw ? pw.vw[wo] : 0.0);
+                    pw.vw ? pw.vw[wo] : 0.0, pw.tw ? pw.tw[wo] : 1.0);
     } else {
       const long long F = g.field, n = idx(g, ii, jj, k);
       P = prim_of(ph, U[n], U[F + n], U[2 * F + n], U[3 * F + n], U[4 * F + n]);
@@ -218,8 +289,8 @@ __global__ void __launch_bounds__(256) k_tauwall(Geom g, Phys ph, int n1, int n2
   double acc[1] = {0.0};
   for (long long n = threadIdx.x; n < g.plane; n += blockDim.x) {
     const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
-    // wall values after the boundary update: u = w = 0, mu = 1**0.7
-    const double mu = pow(1.0, 0.7);
+    // wall values after the boundary update: w = 0, mu = T_w**0.7
+    const double mu = pw.mw ? pw.mw[(top ? g.plane : 0) + n] : pow(1.0, 0.7);
     const double dwdx = mu * (0.0 - 0.0) * 0.5 * ph.invdx;
     const long long a = idx(g, ii, jj, top ? N - 1 : 1), b = idx(g, ii, jj, top ? N - 2 : 2);
     const double u2 = U[F + a] / U[a], u3 = U[F + b] / U[b];
@@ -267,8 +338,8 @@ __global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int
       const bool top = kk == n3 - 1;
       const long long wo = (top ? g.plane : 0) + n;
       P = wall_prim(ph, rw[wo], pw.p[(top ? pw.top_off : 0) + n], pw.uw ? pw.uw[wo] : 0.0,
-                    pw.vw ? pw.vw[wo] : 0.0);
-      if (ph.Ls != 0.0)  // periodic-image ring: every duplicate counts
+                    pw.vw ? pw.vw[wo] : 0.0, pw.tw ? pw.tw[wo] : 1.0);
+      if (ph.Ls != 0.0 && !((pw.wflags >> top) & 1))  // periodic-image ring: every duplicate counts
         wgt = ((ii == 0 || ii == g.P1 - 1) ? 2.0 : 1.0) * ((jj == 0 || jj == g.P2 - 1) ? 2.0 : 1.0);
     } else {
       const long long m = idx(g, ii, jj, kk);
@@ -277,6 +348,15 @@ __global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int
     }
     stat_terms(P, tq);
     for (int q = 0; q < kNStat; q++) acc[q] += wgt * tq[q];
+  }
+  if (wall && ((pw.wflags >> (kk != 0)) & 1)) {
+    // study wall: the duplicates as the last wall update left them
+    const int NRG = ring_size(g.P1, g.P2);
+    const double* rg = pw.ring + (long long)(kk != 0) * kRingFields * NRG;
+    for (int q0 = threadIdx.x; q0 < NRG; q0 += blockDim.x) {
+      stat_terms(ring_prim(ph, rg, NRG, q0), tq);
+      for (int q = 0; q < kNStat; q++) acc[q] += tq[q];
+    }
   }
   block_sum<kNStat>(acc, lds);
   if (threadIdx.x == 0) {
@@ -353,12 +433,23 @@ void launch_diag(int n1, int n2, int n3, double dy, const double* Vdev, double* 
 }
 
 void launch_upload(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull, double* UA,
-                   double* UB, double* W, WallP pw, double* rw, double* consts, hipStream_t s) {
+                   double* UB, double* W, WallP pw, double* rw, double* dw, double* consts,
+                   hipStream_t s) {
   const int B = 256;
   hipLaunchKernelGGL(k_upload, dim3((unsigned)((g.field + B - 1) / B)), dim3(B), 0, s, g, ph, n1,
                      n2, Vfull, UA, UB, W, pw, rw);
+  launch_wall_init(g, ph, n1, n2, Vfull, pw, dw, consts, s);
+}
+
+void launch_wall_init(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull,
+                      WallP pw, double* dw, double* consts, hipStream_t s) {
+  if (pw.uw) {
+    const long long nw = 2 * g.plane + 2LL * ring_size(g.P1, g.P2);
+    hipLaunchKernelGGL(k_wall_init, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, g, ph,
+                       n1, n2, Vfull, pw, dw);
+  }
   hipLaunchKernelGGL(k_consts, dim3((g.n3 + 63) / 64), dim3(64), 0, s, g, ph, n1, n2, Vfull,
-                     consts);
+                     consts, pw.wflags);
 }
 
 void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,

@@ -218,9 +218,11 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       u = T(a.uw[wo]);
       v = T(a.vw[wo]);
       w = T(0);
-      Tt = T(1);
-      mu = T(1);
-      p = T(a.pw[wo]);
+      Tt = T(a.tw[wo]);
+      mu = T(a.mw[wo]);
+      // dw: the duplicate-node correction of this node's wall-plane
+      // divergence, as the pressure offset (2/3) mu/Re ddiv it makes in hD
+      p = T((a.wflags >> (t != 0)) & 1 ? a.pw[wo] + a.dw[wo] : a.pw[wo]);
     } else {
       const T r = T(Uo[0]);
       const T ir = T(1) / r;
@@ -270,6 +272,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       for (int q = 0; q < nring; q++) {
         prim(Uin, o_ring[q], t, u, v, w, Tt, p, mu, Uo);
         const int r = r_ring[q];
+        const int zr = zring(r % RX, r / RX);
+        if (zr >= 0) {
+          srz[t % 3][0][zr] = u;
+          srz[t % 3][1][zr] = v;
+          srz[t % 3][2][zr] = w;
+        }
         sl[s][0][r] = u;
         sl[s][1][r] = v;
         sl[s][2][r] = w;
@@ -278,12 +286,6 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
         spm[s][1][r] = mu;
 #pragma unroll
         for (int c = 0; c < 5; c++) sub[c * NR + r] = T(Uo[c]);
-        const int zr = zring(r % RX, r / RX);
-        if (zr >= 0) {
-          srz[t % 3][0][zr] = u;
-          srz[t % 3][1][zr] = v;
-          srz[t % 3][2][zr] = w;
-        }
       }
     } else {
       // past the top wall: shift the windows by one plane of zeros
@@ -605,10 +607,14 @@ __global__ void __launch_bounds__(TX* TY) k_stage_batched(const Inst* __restrict
 }
 
 // Wall values and wall-pressure extrapolation in FP64 for the fused path
-// (main_v1.9.f90:622-716): rw <- gama Ma^2 p_old / T, the wall slip velocity
-// (Navier slip u_w = Ls du/dn with the one-sided coefficients; 0 for
-// no-slip walls), then the new wall pressure from the z-momentum balance at
-// an impervious wall, with the gravity term -Ri rho_w when buoyancy is on.
+// (main_v1.9.f90:622-716): the wall velocity and temperature (no-slip and
+// T = 1, the Navier slip u_w = Ls du/dn, or a study wall condition, see
+// wall_bc), rho_w <- gama Ma^2 p_old / T_w, mu_w = T_w^0.7, then the new
+// wall pressure from the z-momentum balance at an impervious wall, with the
+// gravity term -Ri rho_w when buoyancy is on.  Under a study condition the
+// threads on the domain edges also update the wall's periodic duplicates
+// (the reference's boundary loop covers them at their own coordinates) and
+// the four corners, whose interior columns the reference never refreshes.
 __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, int ii, int jj,
                                           int wall) {
   const Geom& g = a.g;
@@ -621,30 +627,25 @@ __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, in
   const double* __restrict__ U = stage == 1 ? a.UB : a.UA;
   double* __restrict__ pw = a.pw;
   double* __restrict__ rw = a.rw;
-  double* __restrict__ uw = a.uw;
-  double* __restrict__ vw = a.vw;
   const int check = stage == 2, n1 = a.n1, n2 = a.n2;
-  int* halt = a.halt;
-  unsigned long long* blow = a.blow;
   const long long F = g.field, PL = g.plane;
   const int N = g.n3 - 1;
   const long long n2d = (long long)jj * g.P1 + ii;
-  const double pold = pw[wall * PL + n2d];
-  const double r = ph.gM2 * pold;
-  rw[wall * PL + n2d] = r;
-  if (check && r < 0.0) {
+  const int kw = wall ? N : 0;
+  // blow-up flag at reference node (i, j) of this wall (1-based)
+  auto flag = [&](int i, int j) {
     const unsigned long long lin =
-        ((unsigned long long)(wall ? N : 0) * n2 + (unsigned long long)(jj + 1)) * n1 + (ii + 1);
-    atomicMin(blow, lin);
-    atomicCAS(halt, 0, step);
-  }
+        ((unsigned long long)kw * n2 + (unsigned long long)(j - 1)) * n1 + (i - 1);
+    atomicMin(a.blow, lin);
+    atomicCAS(a.halt, 0, step);
+  };
   const int k1 = wall ? N - 1 : 1, k2 = wall ? N - 2 : 2;
   const double c0 = wall ? ph.b1d2c0 : ph.f1d2c0, c1 = wall ? ph.b1d2c1 : ph.f1d2c1,
                c2 = wall ? ph.b1d2c2 : ph.f1d2c2;
   const double e0 = wall ? ph.b2d1c0 : ph.f2d1c0, e1 = wall ? ph.b2d1c1 : ph.f2d1c1,
                e2 = wall ? ph.b2d1c2 : ph.f2d1c2;
   struct P {
-    double r, u, v, w, p, mu;
+    double r, u, v, w, T, p, mu;
   };
   auto prim = [&](int i, int j, int k) {
     const long long n = (long long)k * PL + (long long)j * g.P1 + i;
@@ -654,34 +655,102 @@ __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, in
     q.u = U[F + n] * ir;
     q.v = U[2 * F + n] * ir;
     q.w = U[3 * F + n] * ir;
-    const double T = U[4 * F + n] * ir - ph.ggM * 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w);
-    q.p = q.r * T / ph.gM2;
-    q.mu = pow(T, 0.7);
+    q.T = U[4 * F + n] * ir - ph.ggM * 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w);
+    q.p = q.r * q.T / ph.gM2;
+    q.mu = pow(q.T, 0.7);
     return q;
   };
+  const bool study = (a.wflags >> wall) & 1;
+  const WallSpec& ws = a.ws[wall];
   // slip velocity of a wall node from the interior values above it
   const double sden = wall ? (1.0 + ph.Ls * c0) : (1.0 - ph.Ls * c0);
   const double ssgn = wall ? -ph.Ls : ph.Ls;
   auto slip = [&](double a1, double a2) {
     return ph.Ls == 0.0 ? 0.0 : ssgn * (c1 * a1 + c2 * a2) / sden;
   };
+  // u, v, T and mu of the wall node at reference (i, j) over column q1, q2
+  struct WV {
+    double u, v, T, mu;
+  };
+  auto wallval = [&](int i, int j, const P& q1, const P& q2) {
+    WV o;
+    if (study) {
+      wall_bc(ws, i, j, a.time, a.x1f, a.x2f, n1, n2, c0, c1, c2, q1.u, q2.u, q1.v, q2.v, q1.T,
+              q2.T, o.u, o.v, o.T);
+      o.mu = pow(o.T, 0.7);
+    } else {
+      o.u = slip(q1.u, q2.u);
+      o.v = slip(q1.v, q2.v);
+      o.T = 1.0;
+      o.mu = 1.0;
+    }
+    return o;
+  };
   const int im = wrapi(ii - 1, g.P1), ip = wrapi(ii + 1, g.P1);
   const int jm = wrapi(jj - 1, g.P2), jp = wrapi(jj + 1, g.P2);
   const P xp1 = prim(ip, jj, k1), xp2 = prim(ip, jj, k2), xm1 = prim(im, jj, k1), xm2 = prim(im, jj, k2);
   const P yp1 = prim(ii, jp, k1), yp2 = prim(ii, jp, k2), ym1 = prim(ii, jm, k1), ym2 = prim(ii, jm, k2);
   const P c1p = prim(ii, jj, k1), c2p = prim(ii, jj, k2);
-  // wall node: u = u_slip, v = v_slip, w = 0, T = 1, mu = 1
-  uw[wall * PL + n2d] = slip(c1p.u, c2p.u);
-  vw[wall * PL + n2d] = slip(c1p.v, c2p.v);
-  const double uwp = slip(xp1.u, xp2.u), uwm = slip(xm1.u, xm2.u);
-  const double vwp = slip(yp1.v, yp2.v), vwm = slip(ym1.v, ym2.v);
-  const double dudxdz = ((c0 * uwp + c1 * xp1.mu * xp1.u + c2 * xp2.mu * xp2.u) -
-                         (c0 * uwm + c1 * xm1.mu * xm1.u + c2 * xm2.mu * xm2.u)) *
+  const int ri = ii + 2, rj = jj + 2;  // reference indices of this node
+  const WV w0 = wallval(ri, rj, c1p, c2p);
+  const WV wxp = wallval(ri + 1, rj, xp1, xp2), wxm = wallval(ri - 1, rj, xm1, xm2);
+  const WV wyp = wallval(ri, rj + 1, yp1, yp2), wym = wallval(ri, rj - 1, ym1, ym2);
+  const double pold = pw[wall * PL + n2d];
+  const double r = ph.gM2 * pold / w0.T;
+  rw[wall * PL + n2d] = r;
+  if (check && (r < 0.0 || w0.T < 0.0)) flag(ri, rj);
+  a.uw[wall * PL + n2d] = w0.u;
+  a.vw[wall * PL + n2d] = w0.v;
+  a.tw[wall * PL + n2d] = w0.T;
+  a.mw[wall * PL + n2d] = w0.mu;
+  if (study) {
+    // The wall-plane divergence of an edge node reads the duplicate beyond
+    // the edge, which the stage kernel holds as the periodic image: dc is
+    // the difference (the image is what that node's own update computes
+    // from the same column).
+    double dc = 0.0;
+    if (ii == 0) dc += (wallval(g.P1 + 1, rj, xm1, xm2).u - wxm.u) * 0.5 * ph.invdx;
+    if (ii == g.P1 - 1) dc += (wxp.u - wallval(2, rj, xp1, xp2).u) * 0.5 * ph.invdx;
+    if (jj == 0) dc += (wallval(ri, g.P2 + 1, ym1, ym2).v - wym.v) * 0.5 * ph.invdy;
+    if (jj == g.P2 - 1) dc += (wyp.v - wallval(ri, 2, yp1, yp2).v) * 0.5 * ph.invdy;
+    // at a wall (w = 0) the divergence enters hD only as (2/3) mu/Re div
+    // beside the pressure: store it as that pressure offset
+    a.dw[wall * PL + n2d] = kTwo3rd * w0.mu * ph.invRe * dc;
+    const int NRG = ring_size(g.P1, g.P2);
+    double* rg = a.ring + (long long)wall * kRingFields * NRG;
+    auto put = [&](int slot, const WV& o, int i, int j) {
+      rg[slot] = o.u;
+      rg[NRG + slot] = o.v;
+      rg[2 * NRG + slot] = o.T;
+      const double rd = ph.gM2 * rg[4 * NRG + slot] / o.T;
+      rg[3 * NRG + slot] = rd;
+      if (check && (rd < 0.0 || o.T < 0.0)) flag(i, j);
+    };
+    if (ii == 0) put(jj, wxm, 1, rj);
+    if (ii == g.P1 - 1) put(g.P2 + jj, wxp, n1, rj);
+    if (jj == 0) put(2 * g.P2 + ii, wym, ri, 1);
+    if (jj == g.P2 - 1) put(2 * g.P2 + g.P1 + ii, wyp, ri, n2);
+    const bool cx = ii == 0 || ii == g.P1 - 1, cy = jj == 0 || jj == g.P2 - 1;
+    if (cx && cy) {
+      // corner (ci, cj) over its stale interior column (the file's values)
+      const int ci = ii == 0 ? 1 : n1, cj = jj == 0 ? 1 : n2;
+      auto fv = [&](int c, int k) {
+        return a.vfile[(((long long)c * g.n3 + k) * n2 + (cj - 1)) * n1 + (ci - 1)];
+      };
+      P q1, q2;
+      q1.u = fv(1, k1); q1.v = fv(2, k1); q1.T = fv(5, k1);
+      q2.u = fv(1, k2); q2.v = fv(2, k2); q2.T = fv(5, k2);
+      const int slot = 2 * g.P2 + 2 * g.P1 + (ii == 0 ? 0 : 1) + (jj == 0 ? 0 : 2);
+      put(slot, wallval(ci, cj, q1, q2), ci, cj);
+    }
+  }
+  const double dudxdz = ((c0 * wxp.mu * wxp.u + c1 * xp1.mu * xp1.u + c2 * xp2.mu * xp2.u) -
+                         (c0 * wxm.mu * wxm.u + c1 * xm1.mu * xm1.u + c2 * xm2.mu * xm2.u)) *
                         0.5 * ph.invdx;
-  const double dvdydz = ((c0 * vwp + c1 * yp1.mu * yp1.v + c2 * yp2.mu * yp2.v) -
-                         (c0 * vwm + c1 * ym1.mu * ym1.v + c2 * ym2.mu * ym2.v)) *
+  const double dvdydz = ((c0 * wyp.mu * wyp.v + c1 * yp1.mu * yp1.v + c2 * yp2.mu * yp2.v) -
+                         (c0 * wym.mu * wym.v + c1 * ym1.mu * ym1.v + c2 * ym2.mu * ym2.v)) *
                         0.5 * ph.invdy;
-  const double d2wdz2 = e0 * 0.0 + e1 * c1p.mu * c1p.w + e2 * c2p.mu * c2p.w;
+  const double d2wdz2 = e0 * w0.mu * 0.0 + e1 * c1p.mu * c1p.w + e2 * c2p.mu * c2p.w;
   const double drhow2dz = c1 * c1p.r * c1p.w * c1p.w + c2 * c2p.r * c2p.w * c2p.w;
   double lhs = ph.invRe * (-dudxdz - dvdydz + kTwo3rd * (dudxdz + dvdydz - 2.0 * d2wdz2)) + drhow2dz;
   if (ph.Ri != 0.0) lhs += ph.Ri * r;

@@ -11,12 +11,19 @@ namespace cc {
 constexpr int kNStat = 23;
 
 // Wall pressure storage: bottom wall at pwall[0..plane), top at pwall + top_off.
-// uw, vw: wall slip velocities [2][plane] (NULL: no-slip walls).
+// uw, vw, tw, mw: wall velocities, temperature and mu = T^0.7 [2][plane]
+// (NULL in the reference-order path, whose walls are isothermal no-slip).
+// ring: duplicate nodes of the wall planes [2][kRingFields][ring_size], used
+// for wall w when bit w of wflags is set (a study wall condition).
 struct WallP {
   double* p;
   long long top_off;
-  const double* uw = nullptr;
-  const double* vw = nullptr;
+  double* uw = nullptr;
+  double* vw = nullptr;
+  double* tw = nullptr;
+  double* mw = nullptr;
+  double* ring = nullptr;
+  int wflags = 0;
 };
 
 // ---- cc_strict.hip: reference-order FP64 stage ------------------------------
@@ -39,8 +46,18 @@ struct Inst {
   double* UB;
   double* pw;  // wall pressure [2][plane]
   double* rw;  // wall density [2][plane]
-  double* uw;  // wall slip velocities [2][plane]
+  double* uw;  // wall velocities [2][plane]
   double* vw;
+  double* tw;  // wall temperature [2][plane]
+  double* mw;  // wall mu = T^0.7 [2][plane]
+  double* ring;  // duplicate wall nodes [2][kRingFields][ring_size(P1, P2)]
+  double* dw;    // duplicate-node correction of the wall-plane hD, as a pressure offset [2][plane]
+  const double* vfile;  // the uploaded var(n1,n2,n3,7) (stale corner columns)
+  const double* x1f;    // x1(n1), x2(n2) as read from the solution file
+  const double* x2f;
+  WallSpec ws[2];       // wall conditions (bottom, top)
+  int wflags;           // bit w: wall w has a study condition
+  double time;          // physical time of the step being taken
   const double* jac;
   const double* invJac;
   const double* force;  // tau_L, tau_U, rho_bar(n3)
@@ -77,8 +94,14 @@ void launch_stage_batched(const Inst* d_insts, const int4* d_work, int nwork, un
 
 // ---- cc_common.hip ----------------------------------------------------------
 // Vfull: the reference's var(n1,n2,n3,7) in Fortran order on the device.
+// (dw: the fused path's wall divergence correction [2][plane], or NULL)
 void launch_upload(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull, double* UA,
-                   double* UB, double* W, WallP pw, double* rw, double* consts, hipStream_t s);
+                   double* UB, double* W, WallP pw, double* rw, double* dw, double* consts,
+                   hipStream_t s);
+// wall arrays (uw, vw, tw, mw, dw) and the duplicate ring from the uploaded
+// file, and the constant plane-sum terms for the current wall conditions
+void launch_wall_init(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull,
+                      WallP pw, double* dw, double* consts, hipStream_t s);
 void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double* U,
                      WallP pw, const double* rw, const double* Vfull, double* Vout,
                      hipStream_t s);

@@ -236,6 +236,13 @@ def write_3d(path: str, time: float, nstep: int, x1, x2, x3, var: np.ndarray,
 _HDR = re.compile(r'TITLE\s*="\s*(\S+)\s+(\d+)\s*"')
 
 
+def file_coords(n1: int, n2: int, n3: int):
+    """x1, x2, x3 as the reference reads them back from a solution file: the
+    analytic mesh written with F10.6 (main_v1.9.f90:763-772) and read into
+    the mesh arrays (:226, 254)."""
+    return tuple(np.array([float(f"{v:10.6f}") for v in a]) for a in mesh(n1, n2, n3)[:3])
+
+
 def read_3d(path: str, n1: int | None = None, n2: int | None = None, n3: int | None = None):
     """Read a solution file written by the reference or by write_3d.
 

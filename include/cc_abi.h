@@ -121,6 +121,58 @@ int cc_batch_step(void* batch, int nsteps);
 int cc_batch_tune(void* batch, int trial_steps, double* out);
 void cc_batch_destroy(void* batch);
 
+/* Wall conditions of the reference's parameter studies.  Each study is a
+ * copy of the v1.9 solver whose bottom-wall block of the boundary update is
+ * changed (top wall unchanged):
+ *   s1..s6   v1.9_STWNV/v1.9_s1/codes/main_v1.9_s1.f90:609 (s3: :609-610),
+ *            spanwise travelling wave v = amp sin(kx x + omega t), on odd j
+ *            with phase_odd added;
+ *   rd1..rd3 v1.9_RD/v1.9_rd1/codes/main_v1.9_rd1.f90:611-640, three discs
+ *            of radius R = x2(n2)/10 at x = 2R, y = 2R, 5R, 8R rotating at
+ *            disc_omega;
+ *   g1, gst* v1.9_groov/v1.9_g1/codes/main_v1.9_g1.f90:609-618 and
+ *            v1.9_groov_strat/v1.9_gst1..5: stripes (bands mod(6j/n2,2)==0, diagonal
+ *            bands, or wavy bands) where the one-sided du/dz = dv/dz = grad,
+ *            no-slip elsewhere;
+ *   st1..st4 v1.9_strat/v1.9_st1/codes/main_v1.9_st1.f90:612: fixed wall
+ *            temperature Tw, or (st3, st4) a fixed one-sided dT/dz.
+ * Like the reference, the condition is applied at every wall node including
+ * the periodic duplicates, each at its own file coordinates.  A zero-filled
+ * cc_wall is the reference's isothermal no-slip wall. */
+#define CC_WALL_NOSLIP 0
+#define CC_WALL_WAVE 1
+#define CC_WALL_DISCS 2
+#define CC_WALL_STRIPES 3
+#define CC_STRIPES_BANDS 0
+#define CC_STRIPES_DIAGONAL 1
+#define CC_STRIPES_WAVY 2
+#define CC_WALLT_ONE 0   /* T = 1 */
+#define CC_WALLT_FIXED 1 /* T = Tw */
+#define CC_WALLT_GRAD 2  /* one-sided dT/dz = dTdz */
+typedef struct {
+  int vel;      /* CC_WALL_* */
+  int stripes;  /* CC_STRIPES_*, for CC_WALL_STRIPES */
+  int temp;     /* CC_WALLT_* */
+  int reserved;
+  double amp, kx, omega, phase_odd; /* travelling wave */
+  double disc_omega;                /* rotating discs */
+  double grad;                      /* stripes: du/dz = dv/dz on the stripes */
+  double Tw, dTdz;                  /* wall temperature or its gradient */
+} cc_wall;
+
+/* Set the condition of wall 0 (bottom, k = 1) or 1 (top, k = n3).  x1 (n1)
+ * and x2 (n2) are the coordinates read from the solution file (the studies
+ * evaluate their conditions at those F10.6 values); NULL keeps the analytic
+ * mesh.  Call before cc_upload, or right after it before any step.  The
+ * FP64 and mixed modes support it; the reference-order mode, batch members
+ * and a non-zero slip length do not (CC_EINVAL). */
+int cc_set_wall(void* handle, int wall, const cc_wall* w, const double* x1, const double* x2);
+
+/* The condition a named study applies ("s1".."s6", "rd1".."rd3", "g1",
+ * "gst1".."gst5", "st1".."st4"; "" or "ref" = the reference wall) and the
+ * wall it applies to.  Host-only; needs no device. */
+int cc_wall_preset(const char* name, int* wall, cc_wall* w);
+
 /* ---- learned subgrid closure (no counterpart in the reference, which has no
  * LES) ----------------------------------------------------------------------
  * An 11-64-64-64-9 GELU MLP: inputs the filtered velocity-gradient tensor

@@ -42,6 +42,9 @@ struct orc {
    * number of a buoyancy body force -Ri rho along z, and the Navier slip
    * length of both walls */
   double Ri, Ls;
+  /* wall conditions of the parameter studies (orc_set_wall) */
+  orc_wall wall[2];
+  double *x1f, *x2f;  /* x1, x2 as read from the solution file */
 };
 
 static const double one = 1.0, two = 2.0, zer = 0.0, quar = 0.25, haf = 0.5;
@@ -114,6 +117,10 @@ orc_t *orc_create(int n1, int n2, int n3, double Re, double Mac, double dt,
   for (int k = 1; k < n3; k++) o->invJac[k] = 1.0 / o->jac[k]; /* :110 */
   memcpy(o->x3f, o->x3, n3 * sizeof(double));
   o->x2end = o->x2[n2 - 1];
+  o->x1f = (double *)calloc(n1, sizeof(double));
+  o->x2f = (double *)calloc(n2, sizeof(double));
+  memcpy(o->x1f, o->x1, n1 * sizeof(double));
+  memcpy(o->x2f, o->x2, n2 * sizeof(double));
 
   /* one-sided coefficients, main:59-83 */
   const double *x3 = o->x3;
@@ -168,7 +175,7 @@ orc_t *orc_create(int n1, int n2, int n3, double Re, double Mac, double dt,
 
 void orc_destroy(orc_t *o) {
   if (!o) return;
-  free(o->x1); free(o->x2); free(o->x3); free(o->x3f); free(o->Xi);
+  free(o->x1); free(o->x2); free(o->x3); free(o->x3f); free(o->x1f); free(o->x2f); free(o->Xi);
   free(o->jac); free(o->invJac);
   free(o->solV); free(o->var); free(o->fD); free(o->gD); free(o->hD);
   free(o->F1); free(o->G1); free(o->H1); free(o->F2); free(o->G2);
@@ -180,6 +187,14 @@ void orc_destroy(orc_t *o) {
 void orc_set_x3_file(orc_t *o, const double *x3, double x2end) {
   memcpy(o->x3f, x3, (size_t)o->n3 * sizeof(double));
   o->x2end = x2end;
+}
+
+void orc_set_wall(orc_t *o, int wall, const orc_wall *w, const double *x1, const double *x2) {
+  if (wall < 0 || wall > 1) return;
+  if (w) o->wall[wall] = *w;
+  else memset(&o->wall[wall], 0, sizeof(orc_wall));
+  if (x1) memcpy(o->x1f, x1, o->n1 * sizeof(double));
+  if (x2) memcpy(o->x2f, x2, o->n2 * sizeof(double));
 }
 
 void orc_set_ext(orc_t *o, double Ri, double Ls) {
@@ -487,6 +502,87 @@ static void wall_ring_images(orc_t *o, int c0, int c1) {
   }
 }
 
+/* Polar angle in [0, 2pi) as the rotating-disc studies compute it
+ * (v1.9_RD/v1.9_rd1/codes/comdata.f90:180-193).  That function leaves its
+ * result unset when either argument is zero; the restatement takes the
+ * angle of the axis there (0, pi/2, pi, 3pi/2; 0 at the centre, where the
+ * radius multiplying it is 0). */
+static double myatand(double A1, double A2) {
+  const double pi = 4.0 * atan(1.0);
+  const double tmp = atan(fabs(A2 / A1));
+  if (A1 > 0 && A2 > 0) return tmp;
+  if (A1 < 0 && A2 > 0) return pi - tmp;
+  if (A1 < 0 && A2 < 0) return pi + tmp;
+  if (A1 > 0 && A2 < 0) return 2.0 * pi - tmp;
+  if (A1 == 0 && A2 > 0) return 0.5 * pi;
+  if (A1 == 0 && A2 < 0) return 1.5 * pi;
+  if (A1 < 0 && A2 == 0) return pi;
+  return 0.0;
+}
+
+/* Stripe predicate of the groove studies at node (i,j) (1-based). */
+static int on_stripe(const orc_t *o, const orc_wall *w, int i, int j) {
+  const int n1 = o->n1, n2 = o->n2;
+  const double x = o->x1f[i - 1], y = o->x2f[j - 1];
+  if (w->stripes == 0) /* main_v1.9_g1.f90:609 */
+    return ((6 * j / n2) % 2) == 0;
+  if (w->stripes == 1) { /* main_v1.9_gst3.f90: diagonal bands */
+    const double r = (o->x1f[n1 - 1] + o->x2f[n2 - 1]) / 8.0;
+    return ((int)floor((x - y + o->x2f[n2 - 1]) / r) % 2) == 0;
+  }
+  { /* main_v1.9_gst5.f90: wavy bands, every third */
+    const double r = o->x2f[n2 - 1] / 8.0;
+    return (((int)floor((y - r * sin(x)) / r) + 1) % 3) == 0;
+  }
+}
+
+/* u, v, T of wall node (i,j) under the wall condition of wall kk, from the
+ * current primitives of the two planes above it (the BC blocks of the
+ * parameter studies; see orc_set_wall). */
+static void wall_mode_values(const orc_t *o, int kk, int i, int j, double *u, double *v,
+                             double *T) {
+  const orc_wall *w = &o->wall[kk];
+  const int n3 = o->n3;
+  const int ka = kk ? n3 - 1 : 2, kb = kk ? n3 - 2 : 3;
+  const double c0 = kk ? o->b1d2c0 : o->f1d2c0, c1 = kk ? o->b1d2c1 : o->f1d2c1,
+               c2 = kk ? o->b1d2c2 : o->f1d2c2;
+  const double x = o->x1f[i - 1], y = o->x2f[j - 1];
+  *u = 0.0;
+  *v = 0.0;
+  if (w->vel == ORC_WALL_WAVE) {
+    const double ph = (j % 2 == 1) ? w->phase_odd : 0.0;
+    *v = w->amp * sin(w->kx * x + w->omega * o->time + ph);
+  } else if (w->vel == ORC_WALL_DISCS) {
+    const double R = o->x2f[o->n2 - 1] / 10.0;
+    const double cy[3] = {2.0, 5.0, 8.0};
+    for (int d = 0; d < 3; d++) {
+      const double ax = x - two * R, ay = y - cy[d] * R;
+      const double r = sqrt(ax * ax + ay * ay);
+      if (r < R) {
+        const double th = myatand(ax, ay);
+        *u = -(r * w->disc_omega * sin(th));
+        *v = r * w->disc_omega * cos(th);
+        break;
+      }
+    }
+  } else if (w->vel == ORC_WALL_STRIPES) {
+    if (on_stripe(o, w, i, j)) {
+      *u = (w->grad - c1 * VR(1, i, j, ka) - c2 * VR(1, i, j, kb)) / c0;
+      *v = (w->grad - c1 * VR(2, i, j, ka) - c2 * VR(2, i, j, kb)) / c0;
+    }
+  }
+  if (w->temp == ORC_WALLT_FIXED)
+    *T = w->Tw;
+  else if (w->temp == ORC_WALLT_GRAD)
+    *T = (w->dTdz - c1 * VR(5, i, j, ka) - c2 * VR(5, i, j, kb)) / c0;
+  else
+    *T = 1.0;
+}
+
+static int wall_active(const orc_t *o, int kk) {
+  return o->wall[kk].vel != ORC_WALL_NOSLIP || o->wall[kk].temp != ORC_WALLT_ONE;
+}
+
 /* ---- wall values, mu, wall pressure, kt: main:622-720 ------------------- */
 static void calcBC(orc_t *o) {
   const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
@@ -497,6 +593,16 @@ static void calcBC(orc_t *o) {
         int k = kk ? n3 : 1;
         VR(1, i, j, k) = 0.0; VR(2, i, j, k) = 0.0; VR(3, i, j, k) = 0.0;
         const int upd = i > 1 && i < n1 && j > 1 && j < n2;
+        if (wall_active(o, kk)) {
+          double u, v, T;
+          wall_mode_values(o, kk, i, j, &u, &v, &T);
+          VR(1, i, j, k) = u;
+          VR(2, i, j, k) = v;
+          VR(5, i, j, k) = T;
+          VR(4, i, j, k) = VR(5, i, j, k) + o->ggM * haf * (u * u + v * v + 0.0 * 0.0);
+          VR(0, i, j, k) = gM2 * VR(6, i, j, k) / VR(5, i, j, k);
+          continue;
+        }
         if (o->Ls != 0.0 && upd) {
           /* Navier slip u_w = Ls du/dn with the one-sided coefficients;
            * Ls -> inf is the zero-gradient wall of main_v1.9_g1.f90:611 */

@@ -49,6 +49,35 @@ void orc_set_x3_file(orc_t *o, const double *x3, double x2end);
  * neither); pinned by the closed forms in tests/test_oracle_ext.py. */
 void orc_set_ext(orc_t *o, double Ri, double Ls);
 
+/* Wall conditions of the reference's parameter studies, one per wall
+ * (wall 0 = bottom k=1, 1 = top k=n3; every study changes the bottom wall
+ * only).  A zero-filled struct is the reference's isothermal no-slip wall.
+ *   vel = ORC_WALL_WAVE: v = amp*sin(kx*x1 + omega*time [+ phase_odd on odd j])
+ *         (v1.9_STWNV/v1.9_s1/codes/main_v1.9_s1.f90:609, ..._s3.f90:609-610);
+ *   vel = ORC_WALL_DISCS: three discs of radius x2(n2)/10 centred at
+ *         x = 2R, y = 2R, 5R, 8R turning at disc_omega
+ *         (v1.9_RD/v1.9_rd1/codes/main_v1.9_rd1.f90:611-640);
+ *   vel = ORC_WALL_STRIPES: on the stripes u, v satisfy the one-sided
+ *         du/dz = dv/dz = grad, no-slip elsewhere; stripes = 0 bands
+ *         mod(6j/n2,2)==0 (v1.9_groov/v1.9_g1/codes/main_v1.9_g1.f90:609-618),
+ *         1 diagonal bands (v1.9_groov_strat/v1.9_gst3/codes/main_v1.9_gst3.f90),
+ *         2 wavy bands (..._gst5.f90);
+ *   temp = ORC_WALLT_FIXED: T = Tw (v1.9_strat/v1.9_st1/codes/main_v1.9_st1.f90:612);
+ *   temp = ORC_WALLT_GRAD: the one-sided dT/dz = dTdz
+ *         (v1.9_strat/v1.9_st3/codes/main_v1.9_st3.f90:612).
+ * x1, x2: the coordinates read from the solution file (the BC loop sees the
+ * file's F10.6 values).  The loop covers every node 1..n1 x 1..n2 of the
+ * wall, periodic duplicates included, each at its own coordinates, so the
+ * duplicates can differ from their periodic images. */
+enum { ORC_WALL_NOSLIP = 0, ORC_WALL_WAVE = 1, ORC_WALL_DISCS = 2, ORC_WALL_STRIPES = 3 };
+enum { ORC_WALLT_ONE = 0, ORC_WALLT_FIXED = 1, ORC_WALLT_GRAD = 2 };
+typedef struct {
+  int vel, stripes, temp, reserved;
+  double amp, kx, omega, phase_odd;
+  double disc_omega, grad, Tw, dTdz;
+} orc_wall;
+void orc_set_wall(orc_t *o, int wall, const orc_wall *w, const double *x1, const double *x2);
+
 /* Forcing state (tauxzL, tauxzU, stat1Spc(:,1)); zero until the first
  * statistics step, exactly as the reference's static initialisation. */
 void orc_set_forcing(orc_t *o, double tauL, double tauU, const double *rhobar);

@@ -39,6 +39,7 @@ def lib():
         L.orc_load.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double]
         L.orc_set_x3_file.argtypes = [C.c_void_p, _dp, C.c_double]
         L.orc_set_ext.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_set_wall.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_forcing.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p]
         L.orc_stage.argtypes = [C.c_void_p, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
@@ -59,6 +60,14 @@ def lib():
         L.orc_avg_spatial.argtypes = [C.c_void_p]
         _lib = L
     return _lib
+
+
+class OrcWall(C.Structure):
+    """Wall condition of one wall (cc_oracle.h orc_wall); zero = reference wall."""
+    _fields_ = [("vel", C.c_int), ("stripes", C.c_int), ("temp", C.c_int), ("reserved", C.c_int),
+                ("amp", C.c_double), ("kx", C.c_double), ("omega", C.c_double),
+                ("phase_odd", C.c_double), ("disc_omega", C.c_double), ("grad", C.c_double),
+                ("Tw", C.c_double), ("dTdz", C.c_double)]
 
 
 class Oracle:
@@ -95,6 +104,20 @@ class Oracle:
 
     def set_ext(self, Ri=0.0, Ls=0.0):
         self.L.orc_set_ext(self.h, Ri, Ls)
+
+    def set_wall(self, wall, spec=None, x1=None, x2=None):
+        """Wall condition of wall 0 (bottom) or 1 (top): spec is a dict of
+        OrcWall fields (None = the reference wall); x1, x2 the solution
+        file's coordinates (default: the analytic mesh)."""
+        w = OrcWall(**(spec or {}))
+        keep = []
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data
+        self.L.orc_set_wall(self.h, int(wall), C.byref(w), ptr(x1), ptr(x2))
 
     def set_forcing(self, tauL, tauU, rhobar=None):
         if rhobar is None:

@@ -21,19 +21,21 @@ sys.path.insert(0, os.path.dirname(HERE))
 from compchan3dmpi_amd import formats  # noqa: E402
 
 
-def build(n1, n2, n3, maxstep, nwrite):
-    out = subprocess.run([os.path.join(HERE, "build_ref.sh"), str(n1), str(n2), str(n3),
-                          str(maxstep), str(nwrite)], check=True, capture_output=True, text=True)
+def build(n1, n2, n3, maxstep, nwrite, study=""):
+    args = [os.path.join(HERE, "build_ref.sh"), str(n1), str(n2), str(n3), str(maxstep),
+            str(nwrite)] + ([study] if study else [])
+    out = subprocess.run(args, check=True, capture_output=True, text=True)
     return out.stdout.strip().splitlines()[-1]
 
 
 def run(var, params: formats.Params, maxstep: int, nwrite: int, workdir: str | None = None,
-        timeout: int = 3600):
-    """Run the reference for maxstep steps from var (7, n3, n2, n1).
+        timeout: int = 3600, study: str = ""):
+    """Run the reference (or one of its wall-condition studies, see
+    build_ref.sh) for maxstep steps from var (7, n3, n2, n1).
 
     Returns a dict with the final 3D file's var and the appended outputs."""
     _, n3, n2, n1 = var.shape
-    exe = build(n1, n2, n3, maxstep, nwrite)
+    exe = build(n1, n2, n3, maxstep, nwrite, study)
     root = workdir or tempfile.mkdtemp(prefix="ccref_")
     formats.write_run_dir(root, params, var)
     env = dict(os.environ)

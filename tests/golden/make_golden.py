@@ -13,6 +13,14 @@ Fixtures (all inputs are regenerated from the seed by the tests):
   c1_64x32x48_s100.npz    config 1 after 100 steps (dt=0.0015, istat=10):
                           sha256 of the full var bytes, per-field sums, L2,
                           Linf, three full planes, tauWall/moment1st rows
+  study_<name>_24x20x16_s10.npz
+                          the wall-condition studies (build_ref.sh STUDY)
+                          after 10 steps from the seeded 24x20x16 start,
+                          dt=1e-3, istat=5: sha256 of var, per-field sums,
+                          L2, Linf, the bottom wall plane (7 fields),
+                          tauWall and moment1st rows
+
+    python tests/golden/make_golden.py studies    # only the study fixtures
 """
 from __future__ import annotations
 
@@ -35,7 +43,31 @@ def ic(n1, n2, n3):
     return formats.round_e22_15(formats.laminar_ic(n1, n2, n3))
 
 
+STUDIES = ("s1", "s3", "s4", "s5", "rd1", "rd3", "g1", "gst2", "gst3", "gst5", "st1", "st3")
+STUDY_GRID = (24, 20, 16)
+
+
+def studies():
+    n1, n2, n3 = STUDY_GRID
+    v = ic(n1, n2, n3)
+    for st in STUDIES:
+        r = ref_run.run(v, formats.Params(dt=0.001, istat=5), 10, 10, study=st)
+        var = np.ascontiguousarray(r["var"])
+        np.savez(
+            os.path.join(HERE, f"study_{st}_{n1}x{n2}x{n3}_s10.npz"),
+            sha256=np.frombuffer(hashlib.sha256(var.tobytes()).digest(), dtype=np.uint8),
+            sums=var.reshape(7, -1).sum(axis=1),
+            l2=np.sqrt((var.reshape(7, -1) ** 2).sum(axis=1)),
+            linf=np.abs(var.reshape(7, -1)).max(axis=1),
+            plane_k1=var[:, 0], tauWall=r["tauWall"], moment1st=r["moment1st"],
+        )
+        print("study", st, "written", flush=True)
+
+
 def main():
+    if sys.argv[1:] == ["studies"]:
+        studies()
+        return
     # tiny grid, 1 step, forcing off
     v = ic(12, 8, 16)
     r = ref_run.run(v, formats.Params(dt=0.001, istat=100), 1, 1)
@@ -58,6 +90,7 @@ def main():
         tauWall=r["tauWall"], moment1st=r["moment1st"],
         KE=r["KE"], rhobulk=r["rhobulk"], Ubulk=r["Ubulk"], massflow=r["massflow"],
     )
+    studies()
     print("fixtures written to", HERE)
 
 

@@ -1,0 +1,119 @@
+"""Wall conditions of the reference's parameter studies in the oracle.
+
+Pinned two ways:
+  * against the reference itself: tests/golden/study_*.npz hold the state
+    after 10 steps of each study's own build (oracle/build_ref.sh STUDY:
+    v1.9_RD's halo-corrected mains as they are, the other studies' bottom-wall
+    block spliced into the halo-corrected v1.9 main), and the oracle must
+    reproduce it bit for bit (sha256 of var) with the wall condition that
+    the library's preset table gives for that study;
+  * against the mathematics of each condition: the one-sided wall gradient
+    the stripes and the fixed-gradient wall impose, the rigid-body rotation
+    of the discs, and the untouched top wall.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from compchan3dmpi_amd import formats
+from compchan3dmpi_amd.channel import wall_preset
+import oracle as orc
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+STUDIES = ("s1", "s3", "s4", "s5", "rd1", "rd3", "g1", "gst2", "gst3", "gst5", "st1", "st3")
+N = (24, 20, 16)
+
+
+def run_study(name, steps=10, n=N):
+    n1, n2, n3 = n
+    v0 = formats.round_e22_15(formats.laminar_ic(n1, n2, n3))
+    x1, x2, x3 = formats.file_coords(n1, n2, n3)
+    o = orc.Oracle(n1, n2, n3, dt=0.001, istat=5)
+    o.load(v0, x3_file=x3, x2_end=x2[-1])
+    wall, spec = wall_preset(name)
+    o.set_wall(wall, spec.as_dict(), x1, x2)
+    assert o.step(steps) == 0
+    return o, (x1, x2, x3)
+
+
+@pytest.mark.parametrize("name", STUDIES)
+def test_study_bit_exact_with_reference(name):
+    g = np.load(os.path.join(GOLD, f"study_{name}_{N[0]}x{N[1]}x{N[2]}_s10.npz"))
+    o, _ = run_study(name)
+    var = np.ascontiguousarray(o.var())
+    assert np.array_equal(var[:, 0], g["plane_k1"])
+    assert hashlib.sha256(var.tobytes()).digest() == g["sha256"].tobytes()
+    tau = o.stats()[0]
+    # tauWall.dat is written with F15.7
+    assert np.allclose(tau, g["tauWall"][-1, 1:], rtol=0, atol=1e-7)
+
+
+def one_sided(o, c, kw):
+    """f0 q1 + f1 q2 + f2 q3 at the bottom wall (main_v1.9.f90:59-70)."""
+    coef = np.empty(12)
+    o.L.orc_get_mesh(o.h, coef, np.empty(4 * o.n[2]), np.empty(8))
+    var = o.var()
+    return coef[0] * var[c, 0] + coef[1] * var[c, 1] + coef[2] * var[c, 2], var
+
+
+def test_stripes_impose_the_one_sided_gradient():
+    """g1 (free-slip bands) and gst2 (du/dz = 0.5 on the bands): on the
+    stripes f0 u1 + f1 u2 + f2 u3 = grad; off them u1 = v1 = 0."""
+    for name, grad in (("g1", 0.0), ("gst2", 0.5)):
+        o, _ = run_study(name, steps=3)
+        du, var = one_sided(o, 1, 0)
+        dv, _ = one_sided(o, 2, 0)
+        n2 = o.n[1]
+        j = np.arange(1, n2 + 1)
+        on = ((6 * j) // n2) % 2 == 0
+        assert np.abs(du[on] - grad).max() < 1e-9 * np.abs(du).max() + 1e-12
+        assert np.abs(dv[on] - grad).max() < 1e-9 * np.abs(dv).max() + 1e-12
+        assert np.all(var[1, 0][~on] == 0.0) and np.all(var[2, 0][~on] == 0.0)
+        assert np.any(var[1, 0][on] != 0.0)
+        assert np.all(var[5, 0] == (0.3 if name == "gst2" else 1.0))
+
+
+def test_fixed_gradient_wall_temperature():
+    """st3: f0 T1 + f1 T2 + f2 T3 = 2 at every bottom-wall node."""
+    o, _ = run_study("st3", steps=3)
+    dT, var = one_sided(o, 5, 0)
+    assert np.abs(dT - 2.0).max() < 1e-10
+    assert np.all(var[1:4, 0] == 0.0)
+
+
+def test_discs_rotate_rigidly():
+    """rd1: inside each disc (u, v) = Omega (-(y - yc), x - xc); zero outside;
+    the top wall stays at rest with T = 1."""
+    o, (x1, x2, _) = run_study("rd1", steps=2)
+    var = o.var()
+    R = x2[-1] / 10.0
+    X, Y = np.meshgrid(x1, x2)
+    u = np.zeros_like(X)
+    v = np.zeros_like(X)
+    inside = np.zeros(X.shape, bool)
+    for c in (2.0, 5.0, 8.0):
+        r = np.hypot(X - 2.0 * R, Y - c * R)
+        m = (r < R) & ~inside
+        u[m] = -3.0 * (Y[m] - c * R)
+        v[m] = 3.0 * (X[m] - 2.0 * R)
+        inside |= m
+    assert inside.sum() > 0
+    assert np.abs(var[1, 0] - u).max() < 1e-13
+    assert np.abs(var[2, 0] - v).max() < 1e-13
+    assert np.all(var[1:4, -1] == 0.0) and np.all(var[5, -1] == 1.0)
+
+
+def test_travelling_wave_phase_and_duplicates():
+    """s3: v_w = 0.5 sin(10 x - 3 t [+ 3.14 on odd j]) at the file x of every
+    wall node, periodic duplicates included (x1(1) = 0 and x1(n1) = 4 pi
+    give values that differ from their periodic images)."""
+    o, (x1, x2, _) = run_study("s3", steps=4)
+    var = o.var()
+    t = o.time
+    j = np.arange(1, o.n[1] + 1)[:, None]
+    ph = np.where(j % 2 == 1, float(np.float32(3.14)), 0.0)
+    want = 0.5 * np.sin(10.0 * x1[None, :] - 3.0 * t + ph)
+    assert np.abs(var[2, 0] - want).max() < 1e-15
+    assert var[2, 0, 5, 0] != var[2, 0, 5, -2]  # duplicate i=1 vs its image i=n1-1
