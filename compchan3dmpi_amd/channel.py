@@ -30,7 +30,10 @@ ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_d
                "cc_batch_destroy", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
                "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
                "cc_mlp_destroy", "cc_mlp_last_error", "cc_filter_samples", "cc_set_closure",
-               "cc_set_wall", "cc_wall_preset")
+               "cc_set_wall", "cc_wall_preset", "cc_average", "cc_postprocess",
+               "cc_pp_fluctuation", "cc_pp_qcriterion", "cc_write_solution", "cc_read_solution",
+               "cc_solution_dims", "cc_write_points", "cc_read_points", "cc_format_field",
+               "cc_io_last_error")
 
 # Study names accepted by cc_wall_preset: the reference's wall-condition
 # parameter studies (v1.9_STWNV, v1.9_RD, v1.9_groov, v1.9_groov_strat,
@@ -112,12 +115,19 @@ def load_library():
     L.cc_set_closure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.cc_set_wall.argtypes = [C.c_void_p, C.c_int, C.POINTER(cc_wall), C.c_void_p, C.c_void_p]
     L.cc_wall_preset.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(cc_wall)]
+    L.cc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.POINTER(C.c_int)]
+    L.cc_postprocess.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]
+    L.cc_pp_fluctuation.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.cc_pp_qcriterion.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.cc_last_error.restype = C.c_char_p
     for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
               "cc_blown", "cc_profile", "cc_abi_version", "cc_batch_create", "cc_batch_step",
               "cc_batch_tune", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
               "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
-              "cc_filter_samples", "cc_set_closure", "cc_set_wall", "cc_wall_preset"):
+              "cc_filter_samples", "cc_set_closure", "cc_set_wall", "cc_wall_preset",
+              "cc_average", "cc_postprocess", "cc_pp_fluctuation", "cc_pp_qcriterion"):
         getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -260,6 +270,39 @@ class Channel:
 
         _check(self.L, self.L.cc_set_wall(self.h, int(wall), C.byref(spec), ptr(x1, self.n[0]),
                                           ptr(x2, self.n[1])))
+
+    def average(self, mode=0):
+        """Time averages of the plane statistics (mode 1: restart the
+        accumulation, 2: stop it, 0: query).  Returns (mean (7, n3), rms
+        (7, n3), covar (9, n3), count)."""
+        n3 = self.n[2]
+        mean, rms, cov = np.zeros((7, n3)), np.zeros((7, n3)), np.zeros((9, n3))
+        cnt = C.c_int()
+        _check(self.L, self.L.cc_average(self.h, int(mode), mean.ctypes.data, rms.ctypes.data,
+                                         cov.ctypes.data, C.byref(cnt)))
+        return mean, rms, cov, cnt.value
+
+    def postprocess(self, mean=None, coords=None, fluc=True, q=True):
+        """Fluctuation field (7, n3, n2, n1) and Q-criterion (n3, n2, n1) of
+        the current state on the device; mean (7, n3) or None for the
+        accumulated time average; coords = (x1, x2, x3) for Q."""
+        n1, n2, n3 = self.n
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data
+
+        F = np.empty((7, n3, n2, n1)) if fluc else None
+        Q = np.empty((n3, n2, n1)) if q else None
+        x1, x2, x3 = coords if coords is not None else (None, None, None)
+        _check(self.L, self.L.cc_postprocess(self.h, ptr(mean), ptr(x1), ptr(x2), ptr(x3),
+                                             F.ctypes.data if fluc else None,
+                                             Q.ctypes.data if q else None))
+        return F, Q
 
     def blown(self):
         ijk = (C.c_int * 3)()

@@ -48,6 +48,8 @@ struct Chan {
   double *dwall = nullptr;                    // their divergence correction
   double *x1f = nullptr, *x2f = nullptr;      // file coordinates for the studies
   bool in_batch = false;
+  double *avgsum = nullptr;  // time-average sums of the plane statistics
+  bool averaging = false;
   double *jac = nullptr, *invJac = nullptr, *force = nullptr, *consts = nullptr;
   double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr, *dbuf = nullptr;
   int* halt = nullptr;
@@ -146,7 +148,7 @@ void setup_mesh(Chan& c) {
 void free_chan(Chan* c) {
   if (!c) return;
   double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->uwall, &c->vwall, &c->twall,
-                     &c->mwall, &c->ring, &c->dwall, &c->x1f, &c->x2f, &c->jac,
+                     &c->mwall, &c->ring, &c->dwall, &c->x1f, &c->x2f, &c->avgsum, &c->jac,
                      &c->invJac, &c->force,
                      &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red, &c->dbuf};
   for (double** b : bufs)
@@ -224,9 +226,11 @@ void batch_enqueue_step(Batch* b, int step, bool stats, hipEvent_t e0, hipEvent_
   }
   if (stats)
     for (Chan* c : b->m)
-      if (step % c->prm.istat == 0)
+      if (step % c->prm.istat == 0) {
         launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
                      c->stats, c->halt, step, b->s);
+        if (c->averaging) launch_avg_accum(c->n3, c->stats, c->avgsum, c->halt, step, b->s);
+      }
 }
 
 }  // namespace
@@ -387,9 +391,11 @@ int cc_step(void* handle, int nsteps) {
       }
 #endif
     }
-    if (step % c->prm.istat == 0)
+    if (step % c->prm.istat == 0) {
       launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
                    c->stats, c->halt, step, c->s);
+      if (c->averaging) launch_avg_accum(c->n3, c->stats, c->avgsum, c->halt, step, c->s);
+    }
     c->nstep = step;
     c->time = c->time + c->prm.dt;
   }
@@ -634,6 +640,152 @@ int cc_wall_preset(const char* name, int* wall, cc_wall* w) {
     return fail(CC_EINVAL, "unknown study: " + n);
   }
   *w = s;
+  return 0;
+}
+
+// ---- post-processing ---------------------------------------------------------
+// mean/rms/covar of avg_v1.7.f90:239-283 from accumulated sums s1, s2, cr.
+static void average_of(int n3, const double* sums, double cnt, double* mean, double* rms,
+                       double* covar) {
+  const int pairs[9][2] = {{0, 3}, {0, 5}, {1, 2}, {1, 3}, {2, 3}, {2, 5}, {5, 3}, {6, 3}, {6, 5}};
+  std::vector<double> s1(7 * n3), s2(7 * n3), cr(9 * n3);
+  for (int q = 0; q < 7 * n3; q++) {
+    s1[q] = sums[q] / cnt;
+    s2[q] = sums[7 * n3 + q] / cnt;
+  }
+  for (int q = 0; q < 9 * n3; q++) cr[q] = sums[14 * n3 + q] / cnt;
+  if (mean) std::memcpy(mean, s1.data(), 7 * n3 * sizeof(double));
+  if (rms)
+    for (int q = 0; q < 7 * n3; q++) rms[q] = std::sqrt(s2[q] - s1[q] * s1[q]);
+  if (covar)
+    for (int p = 0; p < 9; p++)
+      for (int k = 0; k < n3; k++)
+        covar[p * n3 + k] = cr[p * n3 + k] - s1[pairs[p][0] * n3 + k] * s1[pairs[p][1] * n3 + k];
+}
+
+int cc_average(void* handle, int mode, double* mean, double* rms, double* covar, int* count) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  const int n3 = c->n3, m = kNStat * n3 + 1;
+  if (mode == 1) {
+    if (!c->avgsum) HIPCHK(hipMalloc((void**)&c->avgsum, m * sizeof(double)));
+    // a small fill, ordered on the instance's stream before the next step
+    std::vector<double> z(m, 0.0);
+    HIPCHK(hipMemcpyAsync(c->avgsum, z.data(), m * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->averaging = true;
+  } else if (mode == 2) {
+    c->averaging = false;
+  } else if (mode != 0) {
+    return fail(CC_EINVAL, "mode must be 0 (query), 1 (start) or 2 (stop)");
+  }
+  std::vector<double> h(m, 0.0);
+  if (c->avgsum) {
+    HIPCHK(hipMemcpyAsync(h.data(), c->avgsum, m * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  const int cnt = (int)h[m - 1];
+  if (count) *count = cnt;
+  if (cnt > 0) average_of(n3, h.data(), (double)cnt, mean, rms, covar);
+  return 0;
+}
+
+namespace {
+// device buffers of one post-processing call, freed on every path
+struct DevBufs {
+  std::vector<double*> p;
+  ~DevBufs() {
+    for (double* q : p) (void)hipFree(q);
+  }
+  double* get(size_t n) {
+    double* q = nullptr;
+    if (hipMalloc((void**)&q, n * sizeof(double)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return q;
+  }
+};
+int grid_ok(int n1, int n2, int n3) {
+  if (n1 < 3 || n2 < 3 || n3 < 3 || n2 > 65535 || n3 > 65535)
+    return fail(CC_EINVAL, "grid must be at least 3 nodes and at most 65535 in y and z");
+  return 0;
+}
+}  // namespace
+
+int cc_pp_fluctuation(int n1, int n2, int n3, const double* var, const double* mean,
+                      double* out) {
+  if (!var || !mean || !out) return fail(CC_EINVAL, "null argument");
+  if (int rc = grid_ok(n1, n2, n3)) return rc;
+  const size_t N = (size_t)n1 * n2 * n3;
+  DevBufs b;
+  double *dv = b.get(7 * N), *dm = b.get(7 * (size_t)n3), *dout = b.get(7 * N);
+  if (!dv || !dm || !dout) return fail(CC_ENOMEM, "device allocation failed");
+  HIPCHK(hipMemcpy(dv, var, 7 * N * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dm, mean, 7 * n3 * sizeof(double), hipMemcpyHostToDevice));
+  launch_fluc(n1, n2, n3, 7, dv, dm, dout, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, dout, 7 * N * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int cc_pp_qcriterion(int n1, int n2, int n3, const double* x, const double* y, const double* z,
+                     const double* U, double* Q) {
+  if (!x || !y || !z || !U || !Q) return fail(CC_EINVAL, "null argument");
+  if (int rc = grid_ok(n1, n2, n3)) return rc;
+  const size_t N = (size_t)n1 * n2 * n3;
+  DevBufs b;
+  double *dU = b.get(3 * N), *dQ = b.get(N), *dx = b.get(n1), *dy = b.get(n2), *dz = b.get(n3);
+  if (!dU || !dQ || !dx || !dy || !dz) return fail(CC_ENOMEM, "device allocation failed");
+  HIPCHK(hipMemcpy(dU, U, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dx, x, n1 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dy, y, n2 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dz, z, n3 * sizeof(double), hipMemcpyHostToDevice));
+  launch_qcrit(n1, n2, n3, dx, dy, dz, dU, dQ, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(Q, dQ, N * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int cc_postprocess(void* handle, const double* mean, const double* x1, const double* x2,
+                   const double* x3, double* fluc, double* Q) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  if (!c->loaded) return fail(CC_ESTATE, "no state uploaded");
+  if (Q && (!x1 || !x2 || !x3)) return fail(CC_EINVAL, "the Q-criterion needs x1, x2, x3");
+  const int n1 = c->n1, n2 = c->n2, n3 = c->n3;
+  if (int rc = grid_ok(n1, n2, n3)) return rc;
+  std::vector<double> m(7 * n3);
+  if (mean) {
+    std::memcpy(m.data(), mean, m.size() * sizeof(double));
+  } else {
+    int cnt = 0;
+    if (int rc = cc_average(c, 0, m.data(), nullptr, nullptr, &cnt)) return rc;
+    if (cnt == 0) return fail(CC_ESTATE, "no mean given and no time average accumulated");
+  }
+  const size_t N = (size_t)n1 * n2 * n3;
+  DevBufs b;
+  double *dm = b.get(7 * (size_t)n3), *dF = b.get(7 * N);
+  if (!dm || !dF) return fail(CC_ENOMEM, "device allocation failed");
+  const double* V = c->Vfile;
+  if (!c->fresh) {
+    if (!c->Vdev) HIPCHK(hipMalloc((void**)&c->Vdev, 7 * N * sizeof(double)));
+    launch_download(c->g, c->ph, n1, n2, c->UA, c->wallp(), c->rw, c->Vfile, c->Vdev, c->s);
+    V = c->Vdev;
+  }
+  HIPCHK(hipMemcpyAsync(dm, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
+  launch_fluc(n1, n2, n3, 7, V, dm, dF, c->s);
+  HIPCHK(hipGetLastError());
+  if (fluc) HIPCHK(hipMemcpyAsync(fluc, dF, 7 * N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  if (Q) {
+    double *dQ = b.get(N), *dx = b.get(n1), *dy = b.get(n2), *dz = b.get(n3);
+    if (!dQ || !dx || !dy || !dz) return fail(CC_ENOMEM, "device allocation failed");
+    HIPCHK(hipMemcpyAsync(dx, x1, n1 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dy, x2, n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dz, x3, n3 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    launch_qcrit(n1, n2, n3, dx, dy, dz, dF + N, dQ, c->s);  // fields 2..4: u', v', w'
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Q, dQ, N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
   return 0;
 }
 

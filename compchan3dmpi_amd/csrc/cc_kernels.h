@@ -116,4 +116,15 @@ void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U
                   const double* rw, const double* consts, double* force, double* stats,
                   const int* halt, int step, hipStream_t s);
 
+// ---- cc_post.hip: post-processing ------------------------------------------
+// sums[23*n3] += stats, sums[23*n3] += 1 (skipped on a halted run)
+void launch_avg_accum(int n3, const double* stats, double* sums, const int* halt, int step,
+                      hipStream_t s);
+// out = var - mean(k) for nf fields of (n1,n2,n3)
+void launch_fluc(int n1, int n2, int n3, int nf, const double* var, const double* mean,
+                 double* out, hipStream_t s);
+// Q-criterion of the velocity fluctuations U[3] on the mesh x, y, z
+void launch_qcrit(int n1, int n2, int n3, const double* x, const double* y, const double* z,
+                  const double* U, double* Q, hipStream_t s);
+
 }  // namespace cc

@@ -13,7 +13,28 @@ type, bind(C) :: cc_params
 	real(c_double) :: Re, Mac, dt, PexVal, PeyVal, PezVal, Ri_b, slip_len
 end type cc_params
 
+! wall condition of a study (cc_wall); filled by cc_wall_preset
+type, bind(C) :: cc_wall
+	integer(c_int) :: vel, stripes, temp, reserved
+	real(c_double) :: amp, kx, omega, phase_odd, disc_omega, grad, Tw, dTdz
+end type cc_wall
+
 interface
+	integer(c_int) function cc_set_wall(h, wall, w, x1, x2) bind(C, name="cc_set_wall")
+		import :: c_int, c_ptr, c_double, cc_wall
+		type(c_ptr), value :: h
+		integer(c_int), value :: wall
+		type(cc_wall), intent(in) :: w
+		real(c_double), intent(in) :: x1(*), x2(*)
+	end function cc_set_wall
+
+	integer(c_int) function cc_wall_preset(name, wall, w) bind(C, name="cc_wall_preset")
+		import :: c_int, c_char, cc_wall
+		character(kind=c_char), intent(in) :: name(*)
+		integer(c_int), intent(out) :: wall
+		type(cc_wall), intent(out) :: w
+	end function cc_wall_preset
+
 	integer(c_int) function cc_create(p, h) bind(C, name="cc_create")
 		import :: c_int, c_ptr, cc_params
 		type(cc_params), intent(in) :: p

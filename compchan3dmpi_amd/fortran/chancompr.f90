@@ -13,7 +13,8 @@
 !        nodes on stdout, ../debug/3D001.dat and stop (:709-743).
 ! The grid is read from the ZONE line of the solution file instead of the
 ! compile-time nP.  Environment (no counterpart in the reference, where
-! these are compile-time): CC_MAXSTEP (default 50000000), CC_PRECISION
+! these are compile-time): CC_WALL (a study's wall condition),
+! CC_MAXSTEP (default 50000000), CC_PRECISION
 ! (fp64 | fp64-strict | mixed, default fp64), CC_NWRITE (default 1000).
 !=======================================================================
 program chanCompr
@@ -36,6 +37,8 @@ character(len=30) :: filename
 character(len=64) :: env
 character(len=200) :: zone
 type(cc_params) :: prm
+type(cc_wall) :: wspec
+integer(c_int) :: wall
 type(c_ptr) :: h
 integer(8) :: clk0, clk1, crate
 logical :: isBlown
@@ -156,6 +159,17 @@ prm%PexVal = PexVal; prm%PeyVal = PeyVal; prm%PezVal = PezVal
 prm%Ri_b = 0.0d0; prm%slip_len = 0.0d0
 rc = cc_create(prm, h)
 if (rc /= 0) call fatal("cc_create")
+! CC_WALL=<study> applies the wall condition of one of the reference's
+! parameter studies (v1.9_STWNV s1..s6, v1.9_RD rd1..rd3, v1.9_groov g1,
+! v1.9_groov_strat gst1..gst5, v1.9_strat st1..st4) at the file's x1, x2
+call get_environment_variable("CC_WALL", env, status=st)
+if (st == 0 .and. len_trim(env) > 0) then
+	rc = cc_wall_preset(trim(env)//c_null_char, wall, wspec)
+	if (rc /= 0) call fatal("cc_wall_preset")
+	rc = cc_set_wall(h, wall, wspec, x1, x2)
+	if (rc /= 0) call fatal("cc_set_wall")
+	write(6,'(2A)') "Wall condition of study ", trim(env)
+endif
 rc = cc_upload(h, var, int(nstep, c_int), time)
 if (rc /= 0) call fatal("cc_upload")
 

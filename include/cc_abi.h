@@ -173,6 +173,67 @@ int cc_set_wall(void* handle, int wall, const cc_wall* w, const double* x1, cons
  * wall it applies to.  Host-only; needs no device. */
 int cc_wall_preset(const char* name, int* wall, cc_wall* w);
 
+/* ---- post-processing (v1.9/codes/avg_v1.7.f90, v1.9/codes/fluc.f90,
+ * v1.8_validation/codes/qInvariant.f90) on the device ---------------------- */
+
+/* Time averages of the plane statistics.  mode 1 restarts the accumulation,
+ * 2 stops it, 0 only queries.  While on, every statistics step (istat) adds
+ * the device's full-precision stat1Spc/stat2Spc/corrSpc to running sums, so
+ * no moment history has to go through text; mean, rms (n3,7) and covar (n3,9)
+ * are then computed as avg_v1.7.f90:239-283 does (mean = sum/count, rms =
+ * sqrt(<q^2> - <q>^2), covariances <ab> - <a><b> in its order rhow, rhoT,
+ * uv, uw, vw, vT, Tw, pw, pT).  Any output pointer may be NULL. */
+int cc_average(void* handle, int mode, double* mean, double* rms, double* covar, int* count);
+
+/* Fluctuation field var - mean(k) of all 7 variables (fluc.f90:55-62) and
+ * the Q-criterion of the velocity fluctuations (qInvariant.f90:26-81) from
+ * an instance's current state, computed on the device.  mean (n3,7); NULL =
+ * the accumulated time average.  fluc (n1,n2,n3,7) and Q (n1,n2,n3) may be
+ * NULL; Q needs the mesh x1, x2, x3 (the solution file's values, as the
+ * reference program reads them from fluc.dat); Q = 0 on boundary nodes. */
+int cc_postprocess(void* handle, const double* mean, const double* x1, const double* x2,
+                   const double* x3, double* fluc, double* Q);
+
+/* The same two computations on host arrays (for fields read from files):
+ * var, out (n1,n2,n3,7); U (n1,n2,n3,3) = u', v', w'; Q (n1,n2,n3). */
+int cc_pp_fluctuation(int n1, int n2, int n3, const double* var, const double* mean,
+                      double* out);
+int cc_pp_qcriterion(int n1, int n2, int n3, const double* x, const double* y, const double* z,
+                     const double* U, double* Q);
+
+/* ---- formatted point files (host, multi-threaded; nthreads <= 0: all cores
+ * up to 16) --------------------------------------------------------------
+ * Byte-identical to the Fortran runtime's output of the reference's writers:
+ * records (3(1X,F10.6),nf(1X,E22.15)[,1X,I1]) in k, j, i order. */
+
+/* A solution file as main_v1.9.f90:759-773 writes it (title with time and
+ * nstep, variable names, ZONE line, a blank record before every k and j);
+ * pattern (n1,n2,n3) adds the study programs' "Pattern" column (NULL: none).
+ * Replaces the reference's record-by-record Fortran write. */
+int cc_write_solution(const char* path, double time, int nstep, int n1, int n2, int n3,
+                      const double* x1, const double* x2, const double* x3, const double* var,
+                      const int* pattern, int nthreads);
+/* Reads one (replaces main_v1.9.f90:210-251): time, nstep from the title
+ * line, x1, x2, x3 and var (n1,n2,n3,7); any output pointer may be NULL. */
+int cc_read_solution(const char* path, int n1, int n2, int n3, double* x1, double* x2,
+                     double* x3, double* var, double* time, int* nstep, int nthreads);
+/* Grid of a solution file from its ZONE k=,j=,i= line. */
+int cc_solution_dims(const char* path, int* n1, int* n2, int* n3);
+/* Any point file: header (the caller's lines, each ending in '\n'), nf
+ * fields (n1,n2,n3,nf), blank_records != 0 for the blank records before each
+ * k and j (fluc.dat) or 0 (qInvariant.dat). */
+int cc_write_points(const char* path, const char* header, int n1, int n2, int n3,
+                    const double* x1, const double* x2, const double* x3, int nf,
+                    const double* fields, const int* pattern, int blank_records, int nthreads);
+/* Reads one: skips nheader lines, then n1*n2*n3 records of 3 + nf (+ 1 when
+ * has_pattern) numbers, blank records ignored; x1..x3 and fields may be NULL. */
+int cc_read_points(const char* path, int nheader, int n1, int n2, int n3, int nf, int has_pattern,
+                   double* x1, double* x2, double* x3, double* fields, int nthreads);
+/* One value as the writers format it: kind 0 = E22.15, 1 = F10.6; out
+ * receives the field and a NUL (at least 23 bytes). */
+int cc_format_field(double x, int kind, char* out);
+const char* cc_io_last_error(void);
+
 /* ---- learned subgrid closure (no counterpart in the reference, which has no
  * LES) ----------------------------------------------------------------------
  * An 11-64-64-64-9 GELU MLP: inputs the filtered velocity-gradient tensor

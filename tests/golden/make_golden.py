@@ -20,7 +20,16 @@ Fixtures (all inputs are regenerated from the seed by the tests):
                           L2, Linf, the bottom wall plane (7 fields),
                           tauWall and moment1st rows
 
+  post_g1_24x20x16_s50.npz
+                          the reference's post-processing chain (oracle/
+                          build_post.sh: avg_v1.7, fluc, qInvariant) on the
+                          g1 study run for 50 steps (istat=5): its moment
+                          files (inputs) and the text of mean/rms/corr/
+                          plotInfo.dat, sha256 of the 3D.dat it read, of
+                          fluc.dat and qInvariant.dat, and Q on plane k=8
+
     python tests/golden/make_golden.py studies    # only the study fixtures
+    python tests/golden/make_golden.py post       # only the post-processing one
 """
 from __future__ import annotations
 
@@ -64,9 +73,45 @@ def studies():
         print("study", st, "written", flush=True)
 
 
+def post():
+    import subprocess
+    n1, n2, n3 = STUDY_GRID
+    v = ic(n1, n2, n3)
+    r = ref_run.run(v, formats.Params(dt=0.001, istat=5), 50, 50, study="g1")
+    root = r["root"]
+    x1, x2, x3 = formats.mesh(n1, n2, n3)[:3]
+    inp = os.path.join(root, "input", "3D.dat")
+    formats.write_3d(inp, r["time"], r["nstep"], x1, x2, x3, r["var"])
+    exe = subprocess.run([os.path.join(ROOT, "oracle", "build_post.sh"), str(n1), str(n2), str(n3)],
+                         check=True, capture_output=True, text=True).stdout.strip().splitlines()[-1]
+    codes = os.path.join(root, "codes")
+    for prog in ("avg.x", "fluc.x", "qinv.x"):
+        subprocess.run([os.path.join(exe, prog)], cwd=codes, check=True, capture_output=True,
+                       input="", text=True)
+    out = os.path.join(root, "output")
+    rd = lambda p: np.frombuffer(open(p, "rb").read(), dtype=np.uint8)
+    sha = lambda p: np.frombuffer(hashlib.sha256(open(p, "rb").read()).digest(), dtype=np.uint8)
+    qv = np.array([float(l.split()[3]) for l in open(os.path.join(out, "qInvariant.dat")).readlines()[3:]])
+    np.savez(
+        os.path.join(HERE, f"post_g1_{n1}x{n2}x{n3}_s50.npz"),
+        moment1st=rd(os.path.join(out, "moment1st.dat")),
+        moment2nd=rd(os.path.join(out, "moment2nd.dat")),
+        momentJoint=rd(os.path.join(out, "momentJoint.dat")),
+        mean=rd(os.path.join(out, "mean.dat")), rms=rd(os.path.join(out, "rms.dat")),
+        corr=rd(os.path.join(out, "corr.dat")), plotInfo=rd(os.path.join(codes, "plotInfo.dat")),
+        sha_input=sha(inp), sha_fluc=sha(os.path.join(out, "fluc.dat")),
+        sha_qinv=sha(os.path.join(out, "qInvariant.dat")),
+        q_k8=qv.reshape(n3, n2, n1)[7],
+    )
+    print("post-processing fixture written", flush=True)
+
+
 def main():
     if sys.argv[1:] == ["studies"]:
         studies()
+        return
+    if sys.argv[1:] == ["post"]:
+        post()
         return
     # tiny grid, 1 step, forcing off
     v = ic(12, 8, 16)
@@ -91,6 +136,7 @@ def main():
         KE=r["KE"], rhobulk=r["rhobulk"], Ubulk=r["Ubulk"], massflow=r["massflow"],
     )
     studies()
+    post()
     print("fixtures written to", HERE)
 
 

@@ -81,3 +81,31 @@ def test_driver_config1_matches_reference_outputs(precision):
         assert dom == ["64", "32", "48", "1", "1", "64", "1", "32", "1", "48"]
     finally:
         shutil.rmtree(root)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("study", ["s3", "g1", "rd1", "st3"])
+def test_driver_study_matches_reference(study):
+    """CC_WALL=<study>: the driver reproduces that study's reference build
+    (tests/golden/study_*.npz, 24x20x16, 10 steps, dt=1e-3, istat=5)."""
+    print(f"\n[case] driver study {study} 24x20x16 10 steps", flush=True)
+    g = np.load(os.path.join(GOLD, f"study_{study}_24x20x16_s10.npz"))
+    root = make_run(24, 20, 16, 0.001, 5)
+    try:
+        p = run_driver(root, {"CC_MAXSTEP": "10", "CC_NWRITE": "10", "CC_WALL": study})
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        assert f"Wall condition of study {study}" in p.stdout
+        out = os.path.join(root, "output")
+        # written at nstep/nwrite = 1, odd: the second file of the alternation
+        t, ns, x1, x2, x3, v = formats.read_3d(os.path.join(out, "3D001_.dat"))
+        assert ns == 10
+        for c in range(7):
+            ref = g["plane_k1"][c]
+            nr = np.linalg.norm(ref)
+            assert np.linalg.norm(v[c, 0] - ref) <= 1e-12 * max(nr, 1e-300), c
+        assert np.allclose(v.reshape(7, -1).sum(axis=1), g["sums"], rtol=1e-12, atol=1e-9)
+        mine = rows(os.path.join(out, "tauWall.dat"))
+        assert mine.shape == g["tauWall"].shape
+        assert np.abs(mine - g["tauWall"]).max() <= 1.01e-7
+    finally:
+        shutil.rmtree(root)

@@ -1,0 +1,159 @@
+"""Post-processing and the formatted point files, on the CPU.
+
+Pinned against the reference's own post-processing programs
+(tests/golden/post_g1_24x20x16_s50.npz, written by avg_v1.7, fluc and
+qInvariant built from the reference sources by oracle/build_post.sh) on the
+g1 study run, and against the Fortran runtime's formatting (flang, when
+present) for the native writer."""
+import hashlib
+import os
+import random
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from compchan3dmpi_amd import fastio, formats, postproc
+from compchan3dmpi_amd.channel import wall_preset
+import oracle as orc
+import post_ref
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "post_g1_24x20x16_s50.npz"))
+N1, N2, N3 = 24, 20, 16
+
+
+def sha(path):
+    return hashlib.sha256(open(path, "rb").read()).digest()
+
+
+@pytest.fixture(scope="module")
+def run_dir():
+    """A run directory holding the reference's moment files and the 3D.dat
+    that its fluc program read, regenerated with the oracle."""
+    root = tempfile.mkdtemp(prefix="ccpost_")
+    for d in ("codes", "output", "input"):
+        os.makedirs(os.path.join(root, d))
+    for name in ("moment1st", "moment2nd", "momentJoint"):
+        open(os.path.join(root, "output", name + ".dat"), "wb").write(GOLD[name].tobytes())
+    v0 = formats.round_e22_15(formats.laminar_ic(N1, N2, N3))
+    x1f, x2f, x3f = formats.file_coords(N1, N2, N3)
+    o = orc.Oracle(N1, N2, N3, dt=0.001, istat=5)
+    o.load(v0, x3_file=x3f, x2_end=x2f[-1])
+    w, spec = wall_preset("g1")
+    o.set_wall(w, spec.as_dict(), x1f, x2f)
+    assert o.step(50) == 0
+    x1, x2, x3 = formats.mesh(N1, N2, N3)[:3]
+    formats.write_3d(os.path.join(root, "input", "3D.dat"), o.time, o.nstep, x1, x2, x3, o.var())
+    yield root
+    shutil.rmtree(root)
+
+
+def test_oracle_state_is_the_one_the_reference_post_processed(run_dir):
+    assert sha(os.path.join(run_dir, "input", "3D.dat")) == GOLD["sha_input"].tobytes()
+
+
+def test_average_restatement_and_product_match_avg_v17(run_dir):
+    out = os.path.join(run_dir, "output")
+    _, s1 = post_ref.read_moment_rows(os.path.join(out, "moment1st.dat"), 7)
+    _, s2 = post_ref.read_moment_rows(os.path.join(out, "moment2nd.dat"), 7)
+    _, cr = post_ref.read_moment_rows(os.path.join(out, "momentJoint.dat"), 9)
+    mean, rms, cov = post_ref.average(s1, s2, cr)
+    x3 = formats.mesh(N1, N2, N3)[2]
+    assert post_ref.mean_text(x3, mean).encode() == GOLD["mean"].tobytes()
+    assert post_ref.mean_text(x3, rms).encode() == GOLD["rms"].tobytes()
+    assert post_ref.corr_text(x3, cov).encode() == GOLD["corr"].tobytes()
+    postproc.average(os.path.join(run_dir, "codes"))
+    for name in ("mean", "rms", "corr"):
+        assert open(os.path.join(out, name + ".dat"), "rb").read() == GOLD[name].tobytes(), name
+    assert open(os.path.join(run_dir, "codes", "plotInfo.dat"), "rb").read() == GOLD["plotInfo"].tobytes()
+
+
+def test_fluc_and_q_restatements_with_native_writer(run_dir):
+    """fluc.dat and qInvariant.dat byte-identical to the reference programs'."""
+    out = os.path.join(run_dir, "output")
+    open(os.path.join(out, "mean.dat"), "wb").write(GOLD["mean"].tobytes())
+    t, ns, x1, x2, x3, var = fastio.read_solution(os.path.join(run_dir, "input", "3D.dat"))
+    mean = postproc.read_mean(os.path.join(out, "mean.dat"))
+    F = post_ref.fluc(var, mean)
+    hdr = (f'TITLE ="{t:20.10f}{ns:9d}"\n'
+           'Variables ="x","y","z","rho_prime","U_prime","V_prime","W_prime","E_prime",'
+           '"T_prime","P_prime"\n'
+           f'ZONE k={N3:3d},j={N2:3d},i={N1:3d},DATAPACKING="POINT"\n')
+    p = os.path.join(out, "fluc_oracle.dat")
+    fastio.write_points(p, hdr, x1, x2, x3, F)
+    assert sha(p) == GOLD["sha_fluc"].tobytes()
+    x, y, z, Fr = fastio.read_points(p, N1, N2, N3, 7)
+    Q = post_ref.q_invariant(x, y, z, Fr[1:4])
+    # (the fixture holds the 15 significant digits the file carries)
+    assert np.allclose(Q[7], GOLD["q_k8"], rtol=1e-14, atol=0)
+    hq = (' TITLE ="qInvariant"\n Variables = "x","y","z","Q"\n'
+          f'ZONE i={N1:3d},j={N2:3d},k={N3:3d},DATAPACKING="POINT"\n')
+    p = os.path.join(out, "q_oracle.dat")
+    fastio.write_points(p, hq, x, y, z, Q[None], blank_records=False)
+    assert sha(p) == GOLD["sha_qinv"].tobytes()
+
+
+def test_solution_writer_reader_round_trip(tmp_path):
+    v = formats.round_e22_15(formats.laminar_ic(N1, N2, N3))
+    x1, x2, x3 = formats.mesh(N1, N2, N3)[:3]
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    fastio.write_solution(a, 0.15, 100, x1, x2, x3, v, nthreads=1)
+    fastio.write_solution(b, 0.15, 100, x1, x2, x3, v, nthreads=5)
+    assert a.read_bytes() == b.read_bytes()
+    c = tmp_path / "c.dat"
+    formats.write_3d(str(c), 0.15, 100, x1, x2, x3, v)
+    assert a.read_bytes() == c.read_bytes()
+    t, ns, r1, r2, r3, rv = fastio.read_solution(a, nthreads=3)
+    assert (t, ns) == (0.15, 100)
+    assert np.array_equal(rv, v)
+    assert np.array_equal(r1, np.array([float(f"{q:10.6f}") for q in x1]))
+    assert fastio.dims(a) == (N1, N2, N3)
+    pat = (np.arange(N1 * N2 * N3).reshape(N3, N2, N1) % 4).astype(np.int32)
+    d = tmp_path / "d.dat"
+    fastio.write_solution(d, 0.15, 100, x1, x2, x3, v, pattern=pat)
+    first = d.read_text().splitlines()[5]  # after 3 header lines and 2 blank records
+    assert first.endswith(" 0") and len(first) == 3 * 11 + 7 * 23 + 2
+
+
+def test_reader_rejects_wrong_grid(tmp_path):
+    v = formats.laminar_ic(12, 8, 16)
+    x1, x2, x3 = formats.mesh(12, 8, 16)[:3]
+    p = tmp_path / "s.dat"
+    fastio.write_solution(p, 0.0, 0, x1, x2, x3, v)
+    with pytest.raises(Exception):
+        fastio.read_points(p, 12, 8, 15, 7)
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="no Fortran compiler in this image")
+def test_field_formatting_matches_the_fortran_runtime(tmp_path):
+    """E22.15 and F10.6 of 3000 values (random magnitudes 1e-30..1e4, signs,
+    zeros, 3-digit exponents) against flang's own formatted write."""
+    rnd = random.Random(20261203)
+    vals = [0.0, -0.0, 1.0, -1.5, 0.5, 1e-300, -2.5e-300, 1e100, 123456.789]
+    for _ in range(3000):
+        vals.append(rnd.choice((1, -1)) * 10 ** rnd.uniform(-30, 4) * rnd.random())
+    (tmp_path / "v.txt").write_text("\n".join(repr(v) for v in vals) + "\n")
+    src = tmp_path / "f.f90"
+    src.write_text(f"""program f
+implicit none
+double precision :: v
+integer :: i, st
+open(10, file='{tmp_path}/v.txt')
+do i = 1, {len(vals)}
+  read(10, *, iostat=st) v
+  write(6, '(A,E22.15,A,F10.6,A)') '[', v, '|', v, ']'
+end do
+end program
+""")
+    exe = tmp_path / "f"
+    subprocess.run([FLANG, "-O1", str(src), "-o", str(exe)], check=True, capture_output=True)
+    lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    assert len(lines) == len(vals)
+    for v, line in zip(vals, lines):
+        want = f"[{fastio.format_field(v, 'E22.15')}|{fastio.format_field(v, 'F10.6')}]"
+        assert line == want, (v, line, want)
