@@ -198,6 +198,17 @@ int cc_write_points(const char* path, const char* header, int n1, int n2, int n3
   }
   const int T = std::min(nthreads_of(nthreads), n3);
   std::vector<int> bad(T, 0);
+  // the coordinate fields repeat across records: format each once
+  std::vector<char> fx1((size_t)n1 * 11), fx2((size_t)n2 * 11), fx3((size_t)n3 * 11);
+  auto prefmt = [](std::vector<char>& f, const double* x, int n) {
+    for (int q = 0; q < n; q++) {
+      f[(size_t)q * 11] = ' ';
+      put_F(&f[(size_t)q * 11 + 1], x[q], 10, 6);
+    }
+  };
+  prefmt(fx1, x1, n1);
+  prefmt(fx2, x2, n2);
+  prefmt(fx3, x3, n3);
   auto work = [&](int t) {
     const int k0 = (int)((long long)n3 * t / T), k1 = (int)((long long)n3 * (t + 1) / T);
     // format a bounded number of planes at a time
@@ -213,12 +224,10 @@ int cc_write_points(const char* path, const char* header, int n1, int n2, int n3
           if (bl) *q++ = '\n';
           for (int i = 0; i < n1; i++) {
             const long long n = ((long long)k * n2 + j) * n1 + i;
-            *q++ = ' ';
-            q += put_F(q, x1[i], 10, 6);
-            *q++ = ' ';
-            q += put_F(q, x2[j], 10, 6);
-            *q++ = ' ';
-            q += put_F(q, x3[k], 10, 6);
+            memcpy(q, &fx1[(size_t)i * 11], 11);
+            memcpy(q + 11, &fx2[(size_t)j * 11], 11);
+            memcpy(q + 22, &fx3[(size_t)k * 11], 11);
+            q += 33;
             for (int c = 0; c < nf; c++) {
               *q++ = ' ';
               q += put_E22_15(q, fields[(long long)c * N + n]);

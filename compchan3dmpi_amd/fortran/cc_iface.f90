@@ -35,6 +35,31 @@ interface
 		type(cc_wall), intent(out) :: w
 	end function cc_wall_preset
 
+	! the reference's solution files through the native parallel writer/reader
+	integer(c_int) function cc_write_solution(path, time, nstep, n1, n2, n3, x1, x2, x3, var, &
+			pattern, nthreads) bind(C, name="cc_write_solution")
+		import :: c_int, c_char, c_double, c_ptr
+		character(kind=c_char), intent(in) :: path(*)
+		real(c_double), value :: time
+		integer(c_int), value :: nstep, n1, n2, n3, nthreads
+		real(c_double), intent(in) :: x1(*), x2(*), x3(*), var(*)
+		type(c_ptr), value :: pattern
+	end function cc_write_solution
+
+	integer(c_int) function cc_read_solution(path, n1, n2, n3, x1, x2, x3, var, time, nstep, &
+			nthreads) bind(C, name="cc_read_solution")
+		import :: c_int, c_char, c_double
+		character(kind=c_char), intent(in) :: path(*)
+		integer(c_int), value :: n1, n2, n3, nthreads
+		real(c_double), intent(out) :: x1(*), x2(*), x3(*), var(*)
+		real(c_double), intent(out) :: time
+		integer(c_int), intent(out) :: nstep
+	end function cc_read_solution
+
+	type(c_ptr) function cc_io_last_error() bind(C, name="cc_io_last_error")
+		import :: c_ptr
+	end function cc_io_last_error
+
 	integer(c_int) function cc_create(p, h) bind(C, name="cc_create")
 		import :: c_int, c_ptr, cc_params
 		type(cc_params), intent(in) :: p
@@ -111,5 +136,21 @@ function cc_errmsg() result(msg)
 	allocate(character(len=n) :: msg)
 	msg = transfer(s(1:n), msg)
 end function cc_errmsg
+
+function cc_ioerrmsg() result(msg)
+	character(len=:), allocatable :: msg
+	character(kind=c_char), pointer :: cs(:)
+	type(c_ptr) :: p
+	integer :: n
+	p = cc_io_last_error()
+	call c_f_pointer(p, cs, [512])
+	n = 0
+	do while (n < 512)
+		if (cs(n+1) == c_null_char) exit
+		n = n + 1
+	enddo
+	allocate(character(len=n) :: msg)
+	msg = transfer(cs(1:n), msg)
+end function cc_ioerrmsg
 
 end module cc_iface

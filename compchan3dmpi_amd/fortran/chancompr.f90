@@ -13,7 +13,9 @@
 !        nodes on stdout, ../debug/3D001.dat and stop (:709-743).
 ! The grid is read from the ZONE line of the solution file instead of the
 ! compile-time nP.  Environment (no counterpart in the reference, where
-! these are compile-time): CC_WALL (a study's wall condition),
+! these are compile-time): CC_WALL (a study's wall condition), CC_FASTIO
+! (default 1: solution files through the native parallel reader/writer,
+! byte-identical to the Fortran runtime's; 0: the Fortran runtime),
 ! CC_MAXSTEP (default 50000000), CC_PRECISION
 ! (fp64 | fp64-strict | mixed, default fp64), CC_NWRITE (default 1000).
 !=======================================================================
@@ -38,6 +40,7 @@ character(len=64) :: env
 character(len=200) :: zone
 type(cc_params) :: prm
 type(cc_wall) :: wspec
+logical :: fastio
 integer(c_int) :: wall
 type(c_ptr) :: h
 integer(8) :: clk0, clk1, crate
@@ -78,6 +81,9 @@ if (st == 0) read(env,*) maxstep
 nwrite = 1000
 call get_environment_variable("CC_NWRITE", env, status=st)
 if (st == 0) read(env,*) nwrite
+fastio = .true.
+call get_environment_variable("CC_FASTIO", env, status=st)
+if (st == 0) fastio = trim(env) /= "0"
 prec = CC_FP64
 call get_environment_variable("CC_PRECISION", env, status=st)
 if (st == 0) then
@@ -129,6 +135,18 @@ if (st /= 0) then
 endif
 allocate(var(n1,n2,n3,7), x1(n1), x2(n2), x3(n3))
 allocate(stat1(n3,7), stat2(n3,7), corr(n3,9))
+if (fastio) then
+	close(10)
+	if (resOp == 2) then
+		rc = cc_read_solution(trim(filename)//c_null_char, n1, n2, n3, x1, x2, x3, var, t_c, ns_c, 0)
+	else
+		rc = cc_read_solution(trim(inputPath)//c_null_char, n1, n2, n3, x1, x2, x3, var, t_c, ns_c, 0)
+	endif
+	if (rc /= 0) then
+		write(6,'(2A)') "Error in reading the input file: ", cc_ioerrmsg()
+		stop
+	endif
+else
 do k=1,n3
 	read(10,*)
 	do j=1,n2
@@ -149,6 +167,7 @@ do k=1,n3
 	enddo
 enddo
 close(10)
+endif
 write(6,'(A)') "Input file successfully read"
 
 !--------------------------- device instance -----------------------------
@@ -230,6 +249,14 @@ do while (nstep < maxstep)
 		else
 			filename = "../output/3D001_.dat"
 		endif
+		if (fastio) then
+			rc = cc_write_solution(trim(filename)//c_null_char, time, int(nstep, c_int), n1, n2, n3, &
+				x1, x2, x3, var, c_null_ptr, 0)
+			if (rc /= 0) then
+				write(6,'(2A)') "Error in writing the solution file: ", cc_ioerrmsg()
+				stop
+			endif
+		else
 		open(unit=10, file=trim(filename))
 		write(10,103) 'TITLE ="',time,nstep,'"'
 		write(10,*) 'Variables ="x","y","z","Rho","U","V","W","E","T","P"'
@@ -245,6 +272,7 @@ do while (nstep < maxstep)
 			enddo
 		enddo
 		close(10)
+		endif
 		open(unit=10, file="../output/domInfo.txt")
 		write(10,105) n1, n2, n3
 		write(10,106) 1

@@ -84,6 +84,23 @@ def test_driver_config1_matches_reference_outputs(precision):
 
 
 @pytest.mark.gpu
+def test_driver_native_io_matches_fortran_runtime():
+    """CC_FASTIO=1 (native reader/writer) and CC_FASTIO=0 (the Fortran
+    runtime's formatted read and write) give byte-identical solution files."""
+    print("\n[case] driver 24x20x16 20 steps, native vs Fortran-runtime I/O", flush=True)
+    files = []
+    for fast in ("0", "1"):
+        root = make_run(24, 20, 16, 0.001, 5)
+        try:
+            p = run_driver(root, {"CC_MAXSTEP": "20", "CC_NWRITE": "20", "CC_FASTIO": fast})
+            assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+            files.append(open(os.path.join(root, "output", "3D001_.dat"), "rb").read())
+        finally:
+            shutil.rmtree(root)
+    assert len(files[0]) > 0 and files[0] == files[1]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("study", ["s3", "g1", "rd1", "st3"])
 def test_driver_study_matches_reference(study):
     """CC_WALL=<study>: the driver reproduces that study's reference build
