@@ -6,6 +6,11 @@
  * "main:" is v1.10/codes/main_v1.9.f90 (the halo-corrected copy of v1.9,
  * which is the canonical physics), "comdata:" is v1.9/codes/comdata.f90.
  *
+ * The loop nests over nodes are independent per node and run on OpenMP
+ * threads when built with -fopenmp (the results do not depend on the thread
+ * count); the reductions (wall shear, plane statistics, diagnostics) stay
+ * sequential in the reference's order.
+ *
  * One MPI rank with topology 1x1x1 is restated: periodic x/y exchanges become
  * self-copies over the ranges the reference's derived datatypes cover
  * (main:297-313), the z exchange is a no-op (MPI_PROC_NULL), and the
@@ -14,6 +19,9 @@
 #include "cc_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -298,6 +306,7 @@ static inline double quick(double ue, double uw, double qmm, double qm,
 
 static void calcFluxConv(orc_t *o) {
   const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
+  #pragma omp parallel for schedule(static)
   for (int k = 2; k <= n3 - 1; k++)
     for (int j = 2; j <= n2 - 1; j++)
       for (int i = 2; i <= n1 - 1; i++) {
@@ -369,6 +378,7 @@ static void calcFluxVisc(orc_t *o) {
   const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
   const double invRe = o->invRe, ggM = o->ggM;
   const double invdx = o->invdx, invdy = o->invdy, invdXi = o->invdXi;
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 2; j <= n2 - 1; j++)
       for (int i = 2; i <= n1 - 1; i++) {
@@ -431,6 +441,7 @@ static void calcFluxVisc(orc_t *o) {
 /* ---- divergence, source, residual, RK2: main:551-586 ------------------- */
 static void calcSolV(orc_t *o, int iter) {
   const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
+  #pragma omp parallel for schedule(static)
   for (int k = 2; k <= n3 - 1; k++)
     for (int j = 2; j <= n2 - 1; j++)
       for (int i = 2; i <= n1 - 1; i++)
@@ -440,6 +451,7 @@ static void calcSolV(orc_t *o, int iter) {
           A5(o->H2, c, i, j, k) = (A5(o->hD, c, i, j, k + 1) - A5(o->hD, c, i, j, k - 1)) * haf * o->invdXi * o->invJac[k - 1];
         }
   const double tdiff = o->tauL - o->tauU;
+  #pragma omp parallel for schedule(static)
   for (int k = 2; k <= n3 - 1; k++)
     for (int j = 2; j <= n2 - 1; j++)
       for (int i = 2; i <= n1 - 1; i++) {
@@ -471,6 +483,7 @@ static void calcSolV(orc_t *o, int iter) {
 /* ---- primitive recovery: main:599-609 ----------------------------------- */
 static void calcPrim(orc_t *o) {
   const int n1 = o->n1, n2 = o->n2, n3 = o->n3;
+  #pragma omp parallel for schedule(static)
   for (int k = 2; k <= n3 - 1; k++)
     for (int j = 2; j <= n2 - 1; j++)
       for (int i = 2; i <= n1 - 1; i++) {
@@ -623,6 +636,7 @@ static void calcBC(orc_t *o) {
     }
   if (o->Ls != 0.0) wall_ring_images(o, 0, 6);
   /* mu over bn-1..en+1 = 1..n (x, y) and all k (main:643) */
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) MU(i, j, k) = pow(VR(5, i, j, k), 0.7);
@@ -686,6 +700,7 @@ static void calcBC(orc_t *o) {
       }
     }
   if (o->Ls != 0.0) wall_ring_images(o, 6, 6);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) KT(i, j, k) = pow(VR(5, i, j, k), 0.7);
@@ -871,3 +886,11 @@ void orc_get_mesh(const orc_t *o, double *coef, double *mesh, double *scal) {
 }
 void orc_get_residual(const orc_t *o, double *R) { memcpy(R, o->R, nsz(o) * 5 * sizeof(double)); }
 void orc_get_blown(const orc_t *o, int *ijk) { memcpy(ijk, o->blown, sizeof o->blown); }
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

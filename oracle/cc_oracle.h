@@ -106,6 +106,8 @@ void orc_get_stats(const orc_t *o, double *tau, double *stat1, double *stat2,
 void orc_get_mesh(const orc_t *o, double *coef, double *mesh, double *scal);
 /* Residual R(n1,n2,n3,5) of the last stage (updated nodes only, 0 elsewhere). */
 void orc_get_residual(const orc_t *o, double *R);
+/* Threads the node loops run on (OpenMP; 1 without it). */
+int orc_threads(void);
 /* Blow-up location of the last orc_step that returned 1: i,j,k (1-based). */
 void orc_get_blown(const orc_t *o, int *ijk);
 

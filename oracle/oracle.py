@@ -36,6 +36,7 @@ def lib():
         L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                  C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_threads.restype = C.c_int
         L.orc_load.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double]
         L.orc_set_x3_file.argtypes = [C.c_void_p, _dp, C.c_double]
         L.orc_set_ext.argtypes = [C.c_void_p, C.c_double, C.c_double]
