@@ -43,6 +43,25 @@ struct Deriv {
   T dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, dTdx, dTdy, dTdz;
 };
 
+// 1/x for the positive normal values of the fused path (density, 1 + m):
+// the hardware reciprocal refined by two Newton steps, within 1 ulp of the
+// quotient for double (an IEEE division costs ~10 instructions); the
+// hardware reciprocal alone (1 ulp) for float.
+template <typename T>
+__device__ __forceinline__ T rcp(T x);
+template <>
+__device__ __forceinline__ double rcp<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+template <>
+__device__ __forceinline__ float rcp<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
 // T**0.7 for T > 0 (mu = kt = T**0.7, main_v1.9.f90:643,720).
 // Double: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
 // s = (m-1)/(m+1), |s| < 0.172, by its odd series to s^21; then
@@ -57,7 +76,7 @@ __device__ __forceinline__ double pow07d(double x) {
     m *= 2.0;
     e -= 1;
   }
-  const double s = (m - 1.0) / (m + 1.0);
+  const double s = (m - 1.0) * rcp<double>(m + 1.0);
   const double s2 = s * s;
   double p = 1.0 / 21.0;
   p = fma(p, s2, 1.0 / 19.0);
@@ -103,6 +122,7 @@ __device__ __forceinline__ float pow07<float>(float x) {
   return exp2f(0.7f * log2f(x));
 }
 
+
 template <typename T, int TX, int TY>
 __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const int step,
                                            const int tile_x, const int tile_y, const int zc) {
@@ -115,6 +135,10 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   constexpr int NUB = (5 * NR > 4 * (NFX + NGY)) ? 5 * NR : 4 * (NFX + NGY);
   constexpr int NZR = 2 * TY + 2 * TX;          // x- and y- ring-1 points
   static_assert(NRING <= 2 * NT, "tile too small for two ring passes");
+  // first threads of the y-ring (gD) and x-ring (fD) point-flux passes
+  constexpr int kRingG0 = 64;
+  constexpr int kRingF0 = 64 + ((2 * TX + 63) / 64) * 64;
+  static_assert(kRingF0 + 2 * TY <= NT, "ring passes do not fit the block");
 
   __shared__ T sl[2][4][NR];    // u, v, w, T of two planes
   __shared__ T spm[2][2][NR];   // p, mu of two planes
@@ -225,7 +249,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       p = T((a.wflags >> (t != 0)) & 1 ? a.pw[wo] + a.dw[wo] : a.pw[wo]);
     } else {
       const T r = T(Uo[0]);
-      const T ir = T(1) / r;
+      const T ir = rcp<T>(r);
       u = T(Uo[1]) * ir;
       v = T(Uo[2]) * ir;
       w = T(Uo[3]) * ir;
@@ -428,8 +452,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 #pragma unroll
         for (int c = 0; c < 4; c++) fx[c * NFX + li] = Fx[c];
       }
-      if (tid < 2 * TY) {
-        const int side = tid / TY, yy = tid % TY;
+      // the x-ring points go to the wave after the y-ring ones, so that the
+      // two extra passes run on different SIMDs (waves are dealt to the
+      // SIMDs round-robin) instead of both on wave 0
+      const int rf = tid - kRingF0;
+      if (rf >= 0 && rf < 2 * TY) {
+        const int side = rf / TY, yy = rf % TY;
         const int rx = side ? TX + 2 : 1, ry = yy + 2;
         const int r = ry * RX + rx;
         const int zr = zring(rx, ry);
@@ -459,8 +487,9 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 #pragma unroll
         for (int c = 0; c < 4; c++) gy[c * NGY + li] = Gy[c];
       }
-      if (tid < 2 * TX) {
-        const int side = tid / TX, xx = tid % TX;
+      const int rg = tid - kRingG0;
+      if (rg >= 0 && rg < 2 * TX) {
+        const int side = rg / TX, xx = rg % TX;
         const int ry = side ? TY + 2 : 1, rx = xx + 2;
         const int r = ry * RX + rx;
         const int zr = zring(rx, ry);
@@ -522,7 +551,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       const double tdiff = a.force[0] - a.force[1];
       const double rb = a.force[2 + q];
       const double src1 = 0.5 * tdiff * rb;
-      const double src4 = ph.ggM * (double)(T(Uw[2][1]) / T(Uw[2][0])) * 0.5 * tdiff * rb;
+      // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
+      const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
       R[0] = -(double)(Sg[0][0] + H[0]);
 #pragma unroll
       for (int c = 1; c < 5; c++)
@@ -548,10 +578,9 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 #pragma unroll
         for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
         if (stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
-          const double ir = 1.0 / Un[0];
-          const double uu = Un[1] * ir, vv = Un[2] * ir, ws = Un[3] * ir;
-          const double Tn = Un[4] * ir - ph.ggM * 0.5 * (uu * uu + vv * vv + ws * ws);
-          if (Un[0] < 0.0 || Tn < 0.0) {
+          // with rho > 0, T < 0 <=> rho^2 T = rho (rho E) - ggM/2 |rho u|^2 < 0
+          const double r2T = Un[0] * Un[4] - ph.ggM * 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]);
+          if (Un[0] < 0.0 || r2T < 0.0) {
             const unsigned long long lin =
                 ((unsigned long long)q * a.n2 + (unsigned long long)(gj_own + 1)) * a.n1 + (gi_own + 1);
             atomicMin(a.blow, lin);
