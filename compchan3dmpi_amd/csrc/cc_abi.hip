@@ -52,6 +52,7 @@ struct Chan {
   bool averaging = false;
   double *jac = nullptr, *invJac = nullptr, *force = nullptr, *consts = nullptr;
   double *stats = nullptr, *Vfile = nullptr, *Vdev = nullptr, *red = nullptr, *dbuf = nullptr;
+  double *spart = nullptr;  // partial sums of the statistics passes
   int* halt = nullptr;
   unsigned long long* blow = nullptr;
   int nstep = 0;
@@ -150,7 +151,7 @@ void free_chan(Chan* c) {
   double** bufs[] = {&c->UA, &c->UB, &c->W, &c->rw, &c->pw, &c->uwall, &c->vwall, &c->twall,
                      &c->mwall, &c->ring, &c->dwall, &c->x1f, &c->x2f, &c->avgsum, &c->jac,
                      &c->invJac, &c->force,
-                     &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red, &c->dbuf};
+                     &c->consts, &c->stats, &c->Vfile, &c->Vdev, &c->red, &c->dbuf, &c->spart};
   for (double** b : bufs)
     if (*b) (void)hipFree(*b);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -228,7 +229,7 @@ void batch_enqueue_step(Batch* b, int step, bool stats, hipEvent_t e0, hipEvent_
     for (Chan* c : b->m)
       if (step % c->prm.istat == 0) {
         launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
-                     c->stats, c->halt, step, b->s);
+                     c->stats, c->spart, c->halt, step, b->s);
         if (c->averaging) launch_avg_accum(c->n3, c->stats, c->avgsum, c->halt, step, b->s);
       }
 }
@@ -280,6 +281,7 @@ int cc_create(const cc_params* p, void** handle) {
       al(&c->invJac, n3) != hipSuccess || al(&c->force, 2 + n3) != hipSuccess ||
       al(&c->consts, kNStat * n3) != hipSuccess || al(&c->stats, kNStat * n3 + 2) != hipSuccess ||
       al(&c->Vfile, 7 * full) != hipSuccess || al(&c->red, 4096) != hipSuccess ||
+      al(&c->spart, stats_scratch_size(p->n3)) != hipSuccess ||
       hipMalloc((void**)&c->halt, sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&c->blow, sizeof(unsigned long long)) != hipSuccess ||
       (p->precision == CC_FP64_STRICT && al(&c->W, 7 * F) != hipSuccess)) {
@@ -393,7 +395,7 @@ int cc_step(void* handle, int nsteps) {
     }
     if (step % c->prm.istat == 0) {
       launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
-                   c->stats, c->halt, step, c->s);
+                   c->stats, c->spart, c->halt, step, c->s);
       if (c->averaging) launch_avg_accum(c->n3, c->stats, c->avgsum, c->halt, step, c->s);
     }
     c->nstep = step;

@@ -274,20 +274,22 @@ __device__ __forceinline__ void block_sum(double (&a)[NV], double* lds /* [nwave
   }
 }
 
-// Wall shear (main_v1.9.f90:929-988): block 0 bottom, block 1 top.
-__global__ void __launch_bounds__(256) k_tauwall(Geom g, Phys ph, int n1, int n2,
-                                                 const double* __restrict__ U, WallP pw,
-                                                 double* __restrict__ force,
-                                                 double* __restrict__ tau_out, const int* halt,
-                                                 int step) {
+// Wall shear (main_v1.9.f90:929-988) in two deterministic passes: blocks
+// (b, wall) sum a contiguous range of the wall's nodes, then one thread per
+// wall adds the kTauBlocks partials in order.
+constexpr int kTauBlocks = 32;
+__global__ void __launch_bounds__(256) k_tauwall_part(Geom g, Phys ph, const double* __restrict__ U,
+                                                      WallP pw, double* __restrict__ part,
+                                                      const int* halt, int step) {
   __shared__ double lds[4];
   const int h = *halt;
   if (h != 0 && h <= step) return;
-  const bool top = blockIdx.x == 1;
+  const bool top = blockIdx.y == 1;
   const long long F = g.field;
   const int N = g.n3 - 1;
+  const long long n0 = g.plane * blockIdx.x / kTauBlocks, n1e = g.plane * (blockIdx.x + 1) / kTauBlocks;
   double acc[1] = {0.0};
-  for (long long n = threadIdx.x; n < g.plane; n += blockDim.x) {
+  for (long long n = n0 + threadIdx.x; n < n1e; n += blockDim.x) {
     const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
     // wall values after the boundary update: w = 0, mu = T_w**0.7
     const double mu = pw.mw ? pw.mw[(top ? g.plane : 0) + n] : pow(1.0, 0.7);
@@ -303,34 +305,46 @@ __global__ void __launch_bounds__(256) k_tauwall(Geom g, Phys ph, int n1, int n2
     acc[0] += (dwdx + dudz) * ph.invRe;
   }
   block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) {
-    const double t = acc[0] / (double)((n1 - 2) * (n2 - 2));
-    force[top ? 1 : 0] = t;
-    tau_out[top ? 1 : 0] = t;
-  }
+  if (threadIdx.x == 0) part[blockIdx.y * kTauBlocks + blockIdx.x] = acc[0];
 }
 
-// Plane statistics (main_v1.9.f90:993-1025): one block per plane.  Each
-// unique interior node is weighted by the number of times the reference's
-// all-node sum sees it (1 + an x duplicate + a y duplicate); the stale
-// corners and wall rings come in through C.
-__global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int n2,
-                                                    const double* __restrict__ U, WallP pw,
-                                                    const double* __restrict__ rw,
-                                                    const double* __restrict__ C,
-                                                    double* __restrict__ force,
-                                                    double* __restrict__ stats, const int* halt,
-                                                    int step) {
+__global__ void k_tauwall_final(int n1, int n2, const double* __restrict__ part,
+                                double* __restrict__ force, double* __restrict__ tau_out,
+                                const int* halt, int step) {
+  const int h = *halt;
+  if (h != 0 && h <= step) return;
+  const int w = threadIdx.x;
+  if (w >= 2) return;
+  double s = 0.0;
+  for (int b = 0; b < kTauBlocks; b++) s += part[w * kTauBlocks + b];
+  const double t = s / (double)((n1 - 2) * (n2 - 2));
+  force[w] = t;
+  tau_out[w] = t;
+}
+
+// Plane statistics (main_v1.9.f90:993-1025) in two deterministic passes:
+// blocks (s, kk) sum one of kStatSplit node ranges of plane kk, then one
+// thread per plane adds the partials in order.  Each unique interior node is
+// weighted by the number of times the reference's all-node sum sees it (1 +
+// an x duplicate + a y duplicate); the stale corners and wall rings come in
+// through C, a study wall's duplicates through its ring.
+constexpr int kStatSplit = 8;
+__global__ void __launch_bounds__(256) k_planestats_part(Geom g, Phys ph,
+                                                         const double* __restrict__ U, WallP pw,
+                                                         const double* __restrict__ rw,
+                                                         double* __restrict__ part,
+                                                         const int* halt, int step) {
   __shared__ double lds[4 * kNStat];
   const int h = *halt;
   if (h != 0 && h <= step) return;
-  const int kk = blockIdx.x, n3 = g.n3;
+  const int kk = blockIdx.y, n3 = g.n3, sp = blockIdx.x;
   const bool wall = (kk == 0 || kk == n3 - 1);
   const long long F = g.field;
   double acc[kNStat];
   for (int q = 0; q < kNStat; q++) acc[q] = 0.0;
   double tq[kNStat];
-  for (long long n = threadIdx.x; n < g.plane; n += blockDim.x) {
+  const long long n0 = g.plane * sp / kStatSplit, n1e = g.plane * (sp + 1) / kStatSplit;
+  for (long long n = n0 + threadIdx.x; n < n1e; n += blockDim.x) {
     const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
     Prim P;
     double wgt = 1.0;
@@ -349,7 +363,7 @@ __global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int
     stat_terms(P, tq);
     for (int q = 0; q < kNStat; q++) acc[q] += wgt * tq[q];
   }
-  if (wall && ((pw.wflags >> (kk != 0)) & 1)) {
+  if (wall && sp == 0 && ((pw.wflags >> (kk != 0)) & 1)) {
     // study wall: the duplicates as the last wall update left them
     const int NRG = ring_size(g.P1, g.P2);
     const double* rg = pw.ring + (long long)(kk != 0) * kRingFields * NRG;
@@ -359,10 +373,24 @@ __global__ void __launch_bounds__(256) k_planestats(Geom g, Phys ph, int n1, int
     }
   }
   block_sum<kNStat>(acc, lds);
-  if (threadIdx.x == 0) {
-    const double nXY = (double)(n1 * n2);
-    for (int q = 0; q < kNStat; q++) stats[q * n3 + kk] = (acc[q] + C[q * n3 + kk]) / nXY;
-    force[2 + kk] = (acc[0] + C[kk]) / nXY;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kNStat; q++) part[((long long)q * n3 + kk) * kStatSplit + sp] = acc[q];
+}
+
+__global__ void k_planestats_final(int n1, int n2, int n3, const double* __restrict__ part,
+                                   const double* __restrict__ C, double* __restrict__ force,
+                                   double* __restrict__ stats, const int* halt, int step) {
+  const int h = *halt;
+  if (h != 0 && h <= step) return;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= n3) return;
+  const double nXY = (double)(n1 * n2);
+  for (int q = 0; q < kNStat; q++) {
+    double a = 0.0;
+    for (int sp = 0; sp < kStatSplit; sp++) a += part[((long long)q * n3 + kk) * kStatSplit + sp];
+    const double t = (a + C[q * n3 + kk]) / nXY;
+    stats[q * n3 + kk] = t;
+    if (q == 0) force[2 + kk] = t;
   }
 }
 
@@ -460,14 +488,22 @@ void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double
                      n2, U, pw, rw, Vfull, Vout);
 }
 
+int stats_scratch_size(int n3) { return kNStat * n3 * kStatSplit + 2 * kTauBlocks; }
+
 void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
                   const double* rw, const double* consts, double* force, double* stats,
-                  const int* halt, int step, hipStream_t s) {
-  // stats layout: [23][n3] plane statistics, then tau[2]
-  hipLaunchKernelGGL(k_tauwall, dim3(2), dim3(256), 0, s, g, ph, n1, n2, U, pw, force,
+                  double* part, const int* halt, int step, hipStream_t s) {
+  // stats layout: [23][n3] plane statistics, then tau[2]; part: the plane
+  // partials [23][n3][kStatSplit], then the wall-shear partials [2][kTauBlocks]
+  double* tpart = part + kNStat * g.n3 * kStatSplit;
+  hipLaunchKernelGGL(k_tauwall_part, dim3(kTauBlocks, 2), dim3(256), 0, s, g, ph, U, pw, tpart,
+                     halt, step);
+  hipLaunchKernelGGL(k_tauwall_final, dim3(1), dim3(64), 0, s, n1, n2, tpart, force,
                      stats + kNStat * g.n3, halt, step);
-  hipLaunchKernelGGL(k_planestats, dim3(g.n3), dim3(256), 0, s, g, ph, n1, n2, U, pw, rw, consts,
-                     force, stats, halt, step);
+  hipLaunchKernelGGL(k_planestats_part, dim3(kStatSplit, g.n3), dim3(256), 0, s, g, ph, U, pw, rw,
+                     part, halt, step);
+  hipLaunchKernelGGL(k_planestats_final, dim3((g.n3 + 63) / 64), dim3(64), 0, s, n1, n2, g.n3,
+                     part, consts, force, stats, halt, step);
 }
 
 }  // namespace cc

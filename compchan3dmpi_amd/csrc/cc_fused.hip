@@ -17,7 +17,8 @@
 //      LDS (two plane slots, ping-pong);
 //   2. computes the x and y convective fluxes of plane t;
 //   3. computes the point fluxes fD, gD (tile + 1-node halo) and hD (tile)
-//      of plane t-1, then the x and y divergences of plane t-1;
+//      of plane t-1, then the x and y divergences of plane t-1, and adds
+//      hD(t-1) into the z divergences of planes t-2 and t;
 //   4. completes plane t-2 with its z convective flux and z divergence
 //      (register windows along the column), adds the source and writes the
 //      RK update.
@@ -144,7 +145,6 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   __shared__ T spm[2][2][NR];   // p, mu of two planes
   __shared__ T sub[NUB];        // U(5) of plane t, then fD and gD of plane t-1
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
-  __shared__ T shd[3][4][NT];   // hD of the own column, planes mod 3
 
   const Geom g = a.g;
   const Phys& ph = a.ph;
@@ -429,8 +429,19 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       h[1] = -mR * syz;
       h[2] = pr + t23 * mR * divg - T(2) * mR * D.dwdz;
       h[3] = -gkRP * mu * D.dTdz + ggM * w * pr + dissH;
+      // hD(p) enters the z divergence of plane p-1 (finished by this
+      // iteration's phase 4: Sg[0]) and of plane p+1 (begun in this
+      // iteration: Sg[2]); no plane of hD needs to be kept
+      if (p - 1 >= kb && p - 1 < ke) {
+        const T cm = half * invdXi * T(a.invJac[p - 1]);
 #pragma unroll
-      for (int c = 0; c < 4; c++) shd[p % 3][c][tid] = h[c];
+        for (int c = 0; c < 4; c++) Sg[0][c + 1] += h[c] * cm;
+      }
+      if (p + 1 >= kb && p + 1 < ke) {
+        const T cp = half * invdXi * T(a.invJac[p + 1]);
+#pragma unroll
+        for (int c = 0; c < 4; c++) Sg[2][c + 1] -= h[c] * cp;
+      }
       if (need_fg) {
         const T dissF = ggM * invRe * mu * (t23 * u * divg - T(2) * u * D.dudx - v * sxy - w * sxz);
         const T dissG = ggM * invRe * mu * (-u * sxy + t23 * v * divg - T(2) * v * D.dvdy - w * syz);
@@ -546,19 +557,15 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
           H[c] = (-ww[4] * T(Uw[4][c]) + T(8) * ww[3] * T(Uw[3][c]) - T(8) * ww[1] * T(Uw[1][c]) +
                   ww[0] * T(Uw[0][c])) * (invdXi * iJ * T(kOne12th));
       }
-      // z divergence of hD: hD(q+1) - hD(q-1) = slots 2 and 0
+      // (the z divergence of hD is already in Sg[0])
       double R[5];
       const double tdiff = a.force[0] - a.force[1];
       const double rb = a.force[2 + q];
       const double src1 = 0.5 * tdiff * rb;
       // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
       const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
-      R[0] = -(double)(Sg[0][0] + H[0]);
 #pragma unroll
-      for (int c = 1; c < 5; c++)
-        R[c] = -(double)(Sg[0][c] + H[c] +
-                         (shd[(q + 1) % 3][c - 1][tid] - shd[(q + 2) % 3][c - 1][tid]) *
-                             (half * invdXi * iJ));
+      for (int c = 0; c < 5; c++) R[c] = -(double)(Sg[0][c] + H[c]);
       R[1] += src1;
       R[4] += src4;
       if (ph.Ri != 0.0) {  // buoyancy -Ri rho along z and its work
@@ -686,7 +693,7 @@ __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, in
     q.w = U[3 * F + n] * ir;
     q.T = U[4 * F + n] * ir - ph.ggM * 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w);
     q.p = q.r * q.T / ph.gM2;
-    q.mu = pow(q.T, 0.7);
+    q.mu = pow07d(q.T);  // as the stage kernel evaluates mu at these nodes
     return q;
   };
   const bool study = (a.wflags >> wall) & 1;

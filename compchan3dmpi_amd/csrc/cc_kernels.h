@@ -112,9 +112,11 @@ void launch_diag(int n1, int n2, int n3, double dy, const double* Vdev, double* 
                  const double* x3f, double x2e, double* out, hipStream_t s);
 // tau[0], tau[1] -> force[0], force[1]; plane statistics -> stats[23][n3]
 // and force[2 + kk] = mean density of plane kk.
+// part: scratch of stats_scratch_size(n3) doubles
 void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
                   const double* rw, const double* consts, double* force, double* stats,
-                  const int* halt, int step, hipStream_t s);
+                  double* part, const int* halt, int step, hipStream_t s);
+int stats_scratch_size(int n3);
 
 // ---- cc_post.hip: post-processing ------------------------------------------
 // sums[23*n3] += stats, sums[23*n3] += 1 (skipped on a halted run)
