@@ -104,7 +104,10 @@ def cpu_baseline(var, nsteps):
 
 
 def traffic_from_profiles(precision):
-    path = os.path.join(ROOT, "profiles", f"traffic_stage_{precision}_256x128x192.json")
+    """Calibrated HBM bytes per stage launch measured with rocprofv3 for this
+    grid and precision (profiles/), or None when none was measured."""
+    n1, n2, n3 = NP
+    path = os.path.join(ROOT, "profiles", f"traffic_stage_{precision}_{n1}x{n2}x{n3}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -120,6 +123,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64-strict", "mixed"])
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--grid", default=None,
+                    help="n1,n2,n3 to time instead of the default grid, e.g. one GPU's slab "
+                         "of a sharded run (34,128,192 = 1/8 of the default along x)")
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="steps of the CPU restatement timed for cpu_baseline (0 = skip)")
     args = ap.parse_args()
@@ -129,6 +135,9 @@ def main():
               "(one channel instance per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    global NP
+    if args.grid:
+        NP = tuple(int(x) for x in args.grid.split(","))
     n1, n2, n3 = NP
     nodes = n1 * n2 * n3
     log(f"[bench] seeded IC nP={NP}")
@@ -191,7 +200,7 @@ def main():
         "dtype": "f64" if args.precision != "mixed" else "f32/f64",
         "data": "synthetic",
         "config": {
-            "workload": "channel DNS time step, 256x192x128 (reference nP=[256,128,192]), "
+            "workload": f"channel DNS time step, {n1}x{n3}x{n2} (reference nP=[{n1},{n2},{n3}]), "
                         "Ma 1.5, Re 3000, RK2, hybrid central/QUICK, seeded perturbed laminar start",
             "nP": list(NP), "dt": DT, "istat": ISTAT, "precision": args.precision,
             "mixed": mixed,
