@@ -103,6 +103,58 @@ def cpu_baseline(var, nsteps):
     return rate, t1 - t0, int(o.L.orc_threads())
 
 
+REF_STEPS = 20  # compile-time maxstep of the reference build timed as the CPU baseline
+
+
+def cpu_reference(var):
+    """The reference itself (oracle/_ref, built from its sources by
+    oracle/build_ref.sh, one MPI rank per host core up to 16) on the same
+    grid and start for REF_STEPS steps.  Its rate comes from its own
+    10-step CPU-time line (main_v1.9.f90:1066-1067) of the last, warm block.
+    Returns None where that build or MPI is not present."""
+    import re
+    import subprocess
+    import tempfile
+
+    from compchan3dmpi_amd import fastio
+
+    n1, n2, n3 = NP
+    exe = os.path.join(ROOT, "oracle", "_ref", f"n{n1}x{n2}x{n3}_s{REF_STEPS}_w1000", "main.x")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    ncpu = min(16, len(os.sched_getaffinity(0)))
+    topo = {16: (4, 2, 2), 8: (2, 2, 2), 4: (2, 2, 1), 2: (2, 1, 1)}
+    nr = max(k for k in topo if k <= ncpu) if ncpu >= 2 else 1
+    tp = topo.get(nr, (1, 1, 1))
+    root = tempfile.mkdtemp(prefix="ccref_")
+    try:
+        x1, x2, x3 = formats.mesh(n1, n2, n3)[:3]
+        for d in ("codes", "output", "input", "debug"):
+            os.makedirs(os.path.join(root, d), exist_ok=True)
+        formats.write_param(os.path.join(root, "param.dat"),
+                            formats.Params(dt=DT, istat=ISTAT, topo=tp))
+        fastio.write_solution(os.path.join(root, "input", "3D.dat"), 0.0, 0, x1, x2, x3, var)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        p = subprocess.run([mpiexec, "-n", str(nr), exe], cwd=os.path.join(root, "codes"),
+                           capture_output=True, text=True, timeout=600, env=env)
+        blocks = re.findall(r"CPU Time:\s*([0-9.]+)\s*sec for 10 iters", p.stdout)
+        if p.returncode != 0 or not blocks:
+            log(f"[bench] reference run failed (rc {p.returncode}): {p.stdout[-300:]} {p.stderr[-300:]}")
+            return None
+        sec = float(blocks[-1])
+        return {"value": n1 * n2 * n3 * 2 * 10 / sec, "unit": "cell-updates/s", "cores": nr,
+                "kind": "reference",
+                "sample": f"{REF_STEPS} steps of nP={list(NP)} by the reference (flang build of "
+                          f"v1.10/codes/main_v1.9.f90, {nr} MPI ranks, topology "
+                          f"{tp[0]}x{tp[1]}x{tp[2]}); rate from its own CPU-time line for steps "
+                          f"{REF_STEPS - 9}-{REF_STEPS}: {sec:.3f} s per 10 steps"}
+    finally:
+        import shutil
+
+        shutil.rmtree(root, ignore_errors=True)
+
+
 def traffic_from_profiles(precision):
     """Calibrated HBM bytes per stage launch measured with rocprofv3 for this
     grid and precision (profiles/), or None when none was measured."""
@@ -180,11 +232,15 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps")
-        rate, tc, nth = cpu_baseline(var, args.cpu_steps)
-        cpu = {"value": rate, "unit": "cell-updates/s", "cores": nth, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of nP={list(NP)} FP64 (C restatement, "
-                         f"OpenMP node loops on {nth} threads, {tc:.1f} s)"}
+        log("[bench] cpu baseline: the reference build on the host cores")
+        cpu = cpu_reference(var)
+        if cpu is None:
+            log(f"[bench] cpu baseline: no reference build here; oracle, {args.cpu_steps} steps")
+            rate, tc, nth = cpu_baseline(var, args.cpu_steps)
+            cpu = {"value": rate, "unit": "cell-updates/s", "cores": nth, "kind": "port",
+                   "sample": f"{args.cpu_steps} steps of nP={list(NP)} FP64 (C restatement, "
+                             f"OpenMP node loops on {nth} threads, {tc:.1f} s)"}
+        log(f"[bench] cpu baseline: {cpu['value']:.4e} {cpu['unit']} ({cpu['kind']}, {cpu['cores']} cores)")
 
     out = {
         "metric": METRIC,
