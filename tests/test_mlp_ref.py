@@ -34,3 +34,16 @@ def test_bf16_rounding_is_round_to_nearest_even():
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7, 1.0 + 3 * 2 ** -8, -2.5])
     y = mlp_ref.bf16(x)
     assert np.array_equal(y, np.array([1.0, 1.0, 1.0 + 2 ** -7, 1.0 + 2 ** -6, -2.5]))
+
+
+def test_closure_fallback_decision():
+    """The DNS replaces a coarse run that blew up (non-finite shear) or whose
+    mean skin friction misses the DNS's by more than the tolerance."""
+    from compchan3dmpi_amd import closure
+    cf = 1e-3
+    good = np.array([[1.01e-3, -0.99e-3], [1.0e-3, -1.0e-3]])
+    assert closure.accepted(good, cf)
+    assert not closure.accepted(good * 1.05, cf)
+    assert not closure.accepted(np.array([[np.nan, -1e-3]]), cf)
+    assert not closure.accepted(np.zeros((0, 2)), cf)
+    assert closure.accepted(good * 1.02, cf) and not closure.accepted(good * 1.02, cf, tol=0.01)
