@@ -135,7 +135,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   constexpr int NFG = NFX > NGY ? NFX : NGY;
   constexpr int NUB = (5 * NR > 4 * (NFX + NGY)) ? 5 * NR : 4 * (NFX + NGY);
   constexpr int NZR = 2 * TY + 2 * TX;          // x- and y- ring-1 points
-  static_assert(NRING <= 2 * NT, "tile too small for two ring passes");
+  static_assert(NRING <= NT, "one ring node per thread");
   // first threads of the y-ring (gD) and x-ring (fD) point-flux passes
   constexpr int kRingG0 = 64;
   constexpr int kRingF0 = 64 + ((2 * TX + 63) / 64) * 64;
@@ -232,11 +232,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 
   // primitives of a loaded node; walls take the boundary values
   // (main_v1.9.f90:624-643)
-  auto prim = [&](const double* U, long long off, int t, T& u, T& v, T& w, T& Tt, T& p, T& mu,
-                  double (&Uo)[5]) {
-    const long long n = (long long)t * PL + off;
-#pragma unroll
-    for (int c = 0; c < 5; c++) Uo[c] = U[c * F + n];
+  auto prim = [&](long long off, int t, T& u, T& v, T& w, T& Tt, T& p, T& mu,
+                  const double (&Uo)[5]) {
     if (t == 0 || t == n3 - 1) {
       const long long wo = (t == 0 ? 0 : PL) + off;
       u = T(a.uw[wo]);
@@ -261,6 +258,30 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   };
 
   const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
+  const int tlast = min(n3 - 1, tend);             // last plane loaded
+  // The mixed kernel has registers to spare (FP64 runs at 256): it loads
+  // the next plane's state after phase 3 and stage 2's base state at the
+  // top of the iteration, so that a phase of arithmetic hides each global
+  // load's latency.  FP64 loads both where they are used.
+  constexpr bool kAhead = sizeof(T) == 4;
+  double Un1[5], Ur1[5], Ub[5];
+  if constexpr (kAhead) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      Ub[c] = 0.0;
+      Ur1[c] = 0.0;
+    }
+  }
+  auto load_plane = [&](int t) {
+    const long long n = (long long)t * PL;
+#pragma unroll
+    for (int c = 0; c < 5; c++) Un1[c] = Uin[c * F + n + o_own];
+    if (nring > 0) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) Ur1[c] = Uin[c * F + n + o_ring[0]];
+    }
+  };
+  if constexpr (kAhead) load_plane(tb);
   for (int t = tb; t <= tend; t++) {
     const int s = t & 1;
     const bool have_plane = t <= n3 - 1;
@@ -268,10 +289,25 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     // plane t-2 of the own column leaves the LDS slot this iteration
 #pragma unroll
     for (int c = 0; c < 4; c++) zU[c] = sl[s][c][r_own];
+    if constexpr (kAhead) {
+      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
+        const long long nb = (long long)(t - 2) * PL + o_own;
+#pragma unroll
+        for (int c = 0; c < 5; c++) Ub[c] = Ubase[c * F + nb];
+      }
+    }
     if (have_plane) {
       T u, v, w, Tt, p, mu;
       double Uo[5];
-      prim(Uin, o_own, t, u, v, w, Tt, p, mu, Uo);
+      if constexpr (kAhead) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) Uo[c] = Un1[c];
+      } else {
+        const long long n = (long long)t * PL + o_own;
+#pragma unroll
+        for (int c = 0; c < 5; c++) Uo[c] = Uin[c * F + n];
+      }
+      prim(o_own, t, u, v, w, Tt, p, mu, Uo);
       sl[s][0][r_own] = u;
       sl[s][1][r_own] = v;
       sl[s][2][r_own] = w;
@@ -294,7 +330,15 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(a.jac[t]) > T(ph.Pez) * mu;
       zq = (zq << 1) | (qz ? 1u : 0u);
       for (int q = 0; q < nring; q++) {
-        prim(Uin, o_ring[q], t, u, v, w, Tt, p, mu, Uo);
+        if constexpr (kAhead) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Uo[c] = Ur1[c];
+        } else {
+          const long long n = (long long)t * PL + o_ring[q];
+#pragma unroll
+          for (int c = 0; c < 5; c++) Uo[c] = Uin[c * F + n];
+        }
+        prim(o_ring[q], t, u, v, w, Tt, p, mu, Uo);
         const int r = r_ring[q];
         const int zr = zring(r % RX, r / RX);
         if (zr >= 0) {
@@ -523,6 +567,9 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       }
     }
     __syncthreads();
+    if constexpr (kAhead) {
+      if (t + 1 <= tlast) load_plane(t + 1);
+    }
     if (need_fg) {
       const T* fx = sub;
       const T* gy = sub + 4 * NFX;
@@ -580,7 +627,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
           for (int c = 0; c < 5; c++) Un[c] = Uw[2][c] + 0.5 * ph.dt * R[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; c++) Un[c] = Ubase[c * F + n] + ph.dt * R[c];
+          for (int c = 0; c < 5; c++) {
+            if constexpr (kAhead)
+              Un[c] = Ub[c] + ph.dt * R[c];
+            else
+              Un[c] = Ubase[c * F + n] + ph.dt * R[c];
+          }
         }
 #pragma unroll
         for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
