@@ -146,3 +146,19 @@ def test_bulk_diagnostics_match_oracle():
         ch.step(10)
         d = ch.diag(x3f, x2e)
         assert np.allclose(d, o.diag(), rtol=1e-12, atol=1e-15), prec
+
+
+@pytest.mark.parametrize("grid", [(7, 9, 6), (37, 21, 11), (70, 19, 23)])
+@pytest.mark.parametrize("precision", ["fp64-strict", "fp64", "mixed"])
+def test_awkward_grids(grid, precision):
+    """Grids that fill no tile evenly (x and y partial tiles, fewer unique
+    nodes than a tile, the thinnest channel n3 = 6) with statistics every
+    step (istat = 1, forcing on from the first step)."""
+    banner(f"grid {grid} 6 steps istat=1 {precision}")
+    o, ch, vg = run_pair(grid, 6, 1e-3, 1, precision)
+    vo = o.var()
+    tol = 1e-5 if precision == "mixed" else 1e-12
+    assert relL2(conserved(vg), conserved(vo)) <= tol
+    tau_g = ch.stats()[0]
+    tau_o = o.stats()[0]
+    assert np.allclose(tau_g, tau_o, rtol=100 * tol, atol=1e-15)
