@@ -63,10 +63,74 @@ int put_F(char* p, double x, int w, int d) {
   return w;
 }
 
+// The 15 correctly rounded significant digits of |x| (x finite, nonzero)
+// and its decimal exponent e10 (|x| ~ d.ddd 10^e10), without printf: |x| is
+// scaled by an exact power of ten in x87 extended precision (64-bit
+// significand, one rounding, relative error <= 2^-64, i.e. an absolute error
+// below 1e-4 on the scaled value in [1e14, 1e15)), then rounded to an
+// integer.  Returns false (the caller uses snprintf) when the scaled value
+// lies within 1e-3 of a rounding tie or the power of ten is not exact in
+// extended precision (|14 - e10| > 27).
+bool digits15(double x, uint64_t& m, int& e10) {
+  static const long double p10[28] = {
+      1e0L,  1e1L,  1e2L,  1e3L,  1e4L,  1e5L,  1e6L,  1e7L,  1e8L,  1e9L,
+      1e10L, 1e11L, 1e12L, 1e13L, 1e14L, 1e15L, 1e16L, 1e17L, 1e18L, 1e19L,
+      1e20L, 1e21L, 1e22L, 1e23L, 1e24L, 1e25L, 1e26L, 1e27L};
+  const long double ax = std::fabs((long double)x);
+  // decimal exponent from the binary one (within 1; corrected below)
+  int e = (int)std::floor(std::ilogb(x) * 0.30102999566398120);
+  for (int pass = 0; pass < 3; pass++) {
+    const int k = 14 - e;
+    if (k > 27 || k < -27) return false;
+    const long double sc = k >= 0 ? ax * p10[k] : ax / p10[-k];
+    if (sc < 1e14L) { e -= 1; continue; }
+    if (sc >= 1e15L) { e += 1; continue; }
+    const long double fl = std::floor(sc);
+    const long double fr = sc - fl;
+    if (fr > 0.499L && fr < 0.501L) return false;
+    m = (uint64_t)fl + (fr > 0.5L ? 1 : 0);
+    if (m == 1000000000000000ull) {
+      m = 100000000000000ull;
+      e += 1;
+    }
+    e10 = e;
+    return true;
+  }
+  return false;
+}
+
 // E22.15 into p[0..22); returns 22.
 int put_E22_15(char* p, double x) {
   const int w = 22;
   char b[64];
+  if (x != 0.0 && std::isfinite(x)) {
+    uint64_t m;
+    int e10;
+    if (digits15(x, m, e10)) {
+      char* q = p;
+      *q++ = std::signbit(x) ? '-' : ' ';
+      *q++ = '0';
+      *q++ = '.';
+      for (int d = 14; d >= 0; d--) {
+        q[d] = (char)('0' + m % 10);
+        m /= 10;
+      }
+      q += 15;
+      const int ex = e10 + 1, ae = ex < 0 ? -ex : ex;
+      if (ae <= 99) {
+        *q++ = 'E';
+        *q++ = ex < 0 ? '-' : '+';
+        *q++ = (char)('0' + ae / 10);
+        *q++ = (char)('0' + ae % 10);
+      } else {
+        *q++ = ex < 0 ? '-' : '+';
+        *q++ = (char)('0' + ae / 100);
+        *q++ = (char)('0' + (ae / 10) % 10);
+        *q++ = (char)('0' + ae % 10);
+      }
+      return w;
+    }
+  }
   if (std::isnan(x) || std::isinf(x)) {
     const char* s = std::isnan(x) ? "NaN" : (x > 0 ? "Inf" : "-Inf");
     const int n = (int)strlen(s);

@@ -157,3 +157,27 @@ end program
     for v, line in zip(vals, lines):
         want = f"[{fastio.format_field(v, 'E22.15')}|{fastio.format_field(v, 'F10.6')}]"
         assert line == want, (v, line, want)
+
+
+def test_e22_15_fast_path_matches_correct_rounding():
+    """The writer's printf-free E22.15 path against correctly rounded %.14E
+    on 120k values: random magnitudes 1e-30..1e30, random bit patterns,
+    powers of ten and their neighbours, and exact ties at the 15th digit
+    (integers ending in 5 at the 16th), which must round half to even."""
+    rng = np.random.default_rng(7)
+    ties = np.array([float(1234567890123455 + 10 * k) for k in range(200)]
+                    + [float(9999999999999995), float(1000000000000005)])
+    vals = np.concatenate([rng.standard_normal(60000) * 10.0 ** rng.uniform(-30, 30, 60000),
+                           np.frombuffer(rng.bytes(8 * 60000), dtype=np.float64),
+                           10.0 ** np.arange(-30, 31), np.nextafter(10.0 ** np.arange(-30, 31), 0),
+                           ties, -ties])
+    vals = vals[np.isfinite(vals) & (vals != 0)]
+    for v in vals:
+        v = float(v)
+        s = "%.14E" % abs(v)
+        ex = int(s[17:]) + 1
+        ae = abs(ex)
+        tail = ("E" + ("-" if ex < 0 else "+") + f"{ae:02d}") if ae <= 99 else \
+            (("-" if ex < 0 else "+") + f"{ae:03d}")
+        want = ("-" if v < 0 else " ") + "0." + s[0] + s[2:16] + tail
+        assert fastio.format_field(v) == want, (v, fastio.format_field(v), want)
