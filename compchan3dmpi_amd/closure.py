@@ -2,6 +2,7 @@
 library's own BF16-MFMA kernels, and the coarse run it drives.
 
     python -m compchan3dmpi_amd.closure [--quick]
+    python -m compchan3dmpi_amd.closure --bench-mlp N   # training throughput
 
 prints one JSON line comparing the coarse run (nP = [64,32,48], the
 64x48x32 grid) with and without the closure against the DNS it was trained
@@ -223,10 +224,53 @@ def experiment(quick=False, log=print):
     return out
 
 
+# Multiply-adds per sample of the BF16 MFMA layers as the kernels run them
+# (inputs padded 11 -> 16, outputs 9 -> 16): forward 16x64 + 64x64 + 64x64 +
+# 64x16; back-propagation through layers 4..2; weight gradients of all four.
+MACS_FWD = 16 * 64 + 64 * 64 + 64 * 64 + 64 * 16
+MACS_BWD = 64 * 16 + 64 * 64 + 64 * 64
+MACS_DW = MACS_FWD
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def bench_mlp(n=1 << 21, epochs=5, batch=8192):
+    """Training throughput of the closure network on device-resident samples:
+    n random samples, `epochs` epochs of minibatch Adam (forward, backward,
+    weight gradients, update).  Returns samples/s and the achieved MFMA
+    TFLOP/s against the dense BF16 peak."""
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((n, 16)).astype(np.float32)
+    T = rng.standard_normal((n, 16)).astype(np.float32)
+    mlp = Mlp(seed=1)
+    mlp.train(X[:batch * 4], T[:batch * 4], epochs=1, batch=batch, lr=1e-3, seed=0)  # warm-up
+    t0 = time.perf_counter()
+    mlp.train(X, T, epochs=epochs, batch=batch, lr=1e-3, seed=0)
+    dt = time.perf_counter() - t0
+    mlp.forward(X)  # inference over all samples: one k_mlp_fwd launch
+    t1 = time.perf_counter()
+    mlp.forward(X)
+    dtf = time.perf_counter() - t1
+    mlp.close()
+    flop = 2.0 * (MACS_FWD + MACS_BWD + MACS_DW) * n * epochs
+    tf = flop / dt / 1e12
+    return {"samples": n, "epochs": epochs, "batch": batch,
+            "train_seconds": dt, "train_samples_per_s": n * epochs / dt,
+            "train_mfma_tflops": tf, "bf16_peak_tflops": BF16_PEAK_TFLOPS,
+            "train_frac": tf / BF16_PEAK_TFLOPS,
+            "forward_seconds": dtf, "forward_samples_per_s": n / dtf,
+            "note": "wall times include the host<->device copies of the samples; "
+                    "kernel times come from a kernel trace"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--bench-mlp", type=int, default=0, metavar="N",
+                    help="time training on N device-resident samples instead")
     a = ap.parse_args()
+    if a.bench_mlp:
+        print(json.dumps(bench_mlp(a.bench_mlp)), flush=True)
+        return
     out = experiment(a.quick, log=lambda *x: print(*x, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(const float* __restrict__ X,
                                                  float* __restrict__ Y, float* __restrict__ Z) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int s = lane & 15, q = lane >> 4;
-  const int s0 = (blockIdx.x * 4 + wave) * 16;
+  const int s0 = (blockIdx.x * (blockDim.x >> 6) + wave) * 16;
   if (s0 >= N) return;  // whole wave
   const int n = s0 + s;
   const bool valid = n < N;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256) k_mlp_bwd(const float* __restrict__ T,
                                                  double* __restrict__ lossp) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int s = lane & 15, q = lane >> 4;
-  const int gw = blockIdx.x * 4 + wave;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + wave;
   const int s0 = gw * 16;
   if (s0 >= N) return;
   const int n = s0 + s;
@@ -179,7 +179,9 @@ __global__ void __launch_bounds__(256) k_mlp_bwd(const float* __restrict__ T,
 
 // Weight gradients dW_k[in][out] = sum_s A_k[s][in] dZ_k[s][out] over one
 // chunk of samples, one wave per 16x16 block of one layer (40 blocks):
-// M = in, N = out, K = samples.  part[chunk][kNP] receives the chunk sums.
+// M = in, N = out, K = samples.  The waves of the first input block of each
+// layer also sum their 16 dZ columns in FP32, the bias gradients.
+// part[chunk][kNP] receives the chunk sums; every entry is written.
 __global__ void __launch_bounds__(64) k_mlp_dw(const float* __restrict__ X,
                                                const int* __restrict__ idx, int N,
                                                const float* __restrict__ Z,
@@ -194,6 +196,7 @@ __global__ void __launch_bounds__(64) k_mlp_dw(const float* __restrict__ X,
   else { layer = 3; ib = blk - 36; ob = 0; }
   const int c0 = blockIdx.y * chunk, c1 = min(N, c0 + chunk);
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
   for (int s0 = c0; s0 < c1; s0 += 16) {
     bf16x4 a, b;
     for (int j = 0; j < 4; j++) {
@@ -210,6 +213,7 @@ __global__ void __launch_bounds__(64) k_mlp_dw(const float* __restrict__ X,
       }
       a[j] = f2bf(av);
       b[j] = f2bf(bv);
+      bsum += bv;
     }
     acc = mfma(a, b, acc);
   }
@@ -217,19 +221,12 @@ __global__ void __launch_bounds__(64) k_mlp_dw(const float* __restrict__ X,
   const int ld = layer == 3 ? 16 : 64;
   for (int j = 0; j < 4; j++)
     part[(size_t)blockIdx.y * kNP + woff + (ib * 16 + 4 * q + j) * ld + ob * 16 + s] = acc[j];
-}
-
-// Bias gradients: column sums of dZ over one chunk (208 columns).
-__global__ void __launch_bounds__(256) k_mlp_db(const float* __restrict__ dZ, int N, int chunk,
-                                                float* __restrict__ part) {
-  const int col = threadIdx.x;
-  if (col >= 208) return;
-  const int c0 = blockIdx.x * chunk, c1 = min(N, c0 + chunk);
-  float s = 0.f;
-  for (int n = c0; n < c1; n++) s += dZ[(size_t)n * 208 + col];
-  const int boff = col < 64 ? kB1 + col : col < 128 ? kB2 + col - 64 : col < 192 ? kB3 + col - 128
-                                                                                 : kB4 + col - 192;
-  part[(size_t)blockIdx.x * kNP + boff] = s;
+  if (ib == 0) {  // wave-uniform
+    bsum += __shfl_xor(bsum, 16, 64);
+    bsum += __shfl_xor(bsum, 32, 64);
+    const int boff = layer == 0 ? kB1 : layer == 1 ? kB2 : layer == 2 ? kB3 : kB4;
+    if (q == 0) part[(size_t)blockIdx.y * kNP + boff + ob * 16 + s] = bsum;
+  }
 }
 
 __device__ __forceinline__ bool frozen(int p) {
@@ -239,15 +236,27 @@ __device__ __forceinline__ bool frozen(int p) {
   return false;
 }
 
-// Sum the chunk partials in a fixed order; then either store the gradient
-// or take an Adam step and refresh the BF16 copy.
-__global__ void k_mlp_reduce(const float* __restrict__ part, int nchunk, float* __restrict__ grad,
-                             float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
-                             short* __restrict__ wb, int adam, float lr, float c1, float c2) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNP) return;
-  float gsum = 0.f;
-  for (int c = 0; c < nchunk; c++) gsum += part[(size_t)c * kNP + p];
+// Sum the chunk partials in a fixed order (four interleaved slices of the
+// chunks per parameter, then the slices in order); then either store the
+// gradient or take an Adam step and refresh the BF16 copy.  256 threads:
+// 64 parameters x 4 slices.
+constexpr int kRedParams = 64;
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float* __restrict__ part, int nchunk,
+                                                    float* __restrict__ grad,
+                                                    float* __restrict__ P, float* __restrict__ m,
+                                                    float* __restrict__ v,
+                                                    short* __restrict__ wb, int adam, float lr,
+                                                    float c1, float c2) {
+  __shared__ float sl[4][kRedParams];
+  const int t = threadIdx.x & (kRedParams - 1), slice = threadIdx.x / kRedParams;
+  const int p = blockIdx.x * kRedParams + t;
+  float acc = 0.f;
+  if (p < kNP)
+    for (int c = slice; c < nchunk; c += 4) acc += part[(size_t)c * kNP + p];
+  sl[slice][t] = acc;
+  __syncthreads();
+  if (slice != 0 || p >= kNP) return;
+  float gsum = ((sl[0][t] + sl[1][t]) + sl[2][t]) + sl[3][t];
   if (frozen(p)) gsum = 0.f;
   if (!adam) {
     grad[p] = gsum;
@@ -467,7 +476,7 @@ struct Mlp {
   float *X = nullptr, *T = nullptr, *Y = nullptr, *Z = nullptr, *dZ = nullptr;
   int* idx = nullptr;
   double* lossp = nullptr;
-  int cap = 0, capIdx = 0, nchunk_cap = 0, step = 0;
+  int cap = 0, capIdx = 0, nchunk_cap = 0, loss_cap = 0, step = 0;
   hipStream_t s = nullptr;
 };
 
@@ -475,11 +484,9 @@ int ensure(Mlp* m, int nbatch, int ndata) {
   if (nbatch > m->cap) {
     for (float** b : {&m->Y, &m->Z, &m->dZ})
       if (*b) MCHK(hipFree(*b));
-    if (m->lossp) MCHK(hipFree(m->lossp));
     MCHK(hipMalloc((void**)&m->Y, (size_t)nbatch * 16 * sizeof(float)));
     MCHK(hipMalloc((void**)&m->Z, (size_t)nbatch * 192 * sizeof(float)));
     MCHK(hipMalloc((void**)&m->dZ, (size_t)nbatch * 208 * sizeof(float)));
-    MCHK(hipMalloc((void**)&m->lossp, (size_t)(nbatch / 16 + 4) * sizeof(double)));
     m->cap = nbatch;
   }
   if (ndata > m->capIdx) {
@@ -494,28 +501,44 @@ int ensure(Mlp* m, int nbatch, int ndata) {
   return 0;
 }
 
-constexpr int kChunk = 4096;
+// Samples per weight-gradient chunk: small enough that a batch of a few
+// thousand samples still gives the gradient kernel thousands of waves, and
+// at most kMaxChunks chunks so that the partials stay within 11 MB.
+constexpr int kMinChunk = 128, kMaxChunks = 256;
+
+struct Chunks {
+  int n, len;
+};
+
+Chunks chunks_for(int N) {
+  const int want = std::min(kMaxChunks, (N + kMinChunk - 1) / kMinChunk);
+  int len = (N + want - 1) / want;
+  len = (len + 15) / 16 * 16;
+  return {(N + len - 1) / len, len};
+}
 
 // forward + backward + gradient partials of one batch (samples idx[b0..b0+nb)
-// of the resident data set); returns the summed squared error
-int batch_grad(Mlp* m, const int* idx, int nb, int nchunk, double* sse) {
-  const int nblk = (nb + 63) / 64;
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->s, m->X, idx, nb, m->wb, m->P, m->Y,
+// of the resident data set); lossp[(nb+15)/16] receives the summed squared
+// error of each 16-sample tile.  One wave per block: a batch of a few
+// thousand samples then still spreads over every CU.
+int batch_grad(Mlp* m, const int* idx, int nb, Chunks ck, double* lossp) {
+  const int nw = (nb + 15) / 16;
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(nw), dim3(64), 0, m->s, m->X, idx, nb, m->wb, m->P, m->Y,
                      m->Z);
-  hipLaunchKernelGGL(k_mlp_bwd, dim3(nblk), dim3(256), 0, m->s, m->T, idx, nb, m->wb, m->Y, m->Z,
-                     m->dZ, m->lossp);
-  hipLaunchKernelGGL(k_mlp_dw, dim3(40, nchunk), dim3(64), 0, m->s, m->X, idx, nb, m->Z, m->dZ,
-                     kChunk, m->part);
-  hipLaunchKernelGGL(k_mlp_db, dim3(nchunk), dim3(256), 0, m->s, m->dZ, nb, kChunk, m->part);
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(nw), dim3(64), 0, m->s, m->T, idx, nb, m->wb, m->Y, m->Z,
+                     m->dZ, lossp);
+  hipLaunchKernelGGL(k_mlp_dw, dim3(40, ck.n), dim3(64), 0, m->s, m->X, idx, nb, m->Z, m->dZ,
+                     ck.len, m->part);
   MCHK(hipGetLastError());
-  if (sse) {
-    const int nw = (nb + 15) / 16;
-    std::vector<double> lp(nw);
-    MCHK(hipMemcpyAsync(lp.data(), m->lossp, nw * sizeof(double), hipMemcpyDeviceToHost, m->s));
-    MCHK(hipStreamSynchronize(m->s));
-    double t = 0.0;
-    for (double x : lp) t += x;
-    *sse = t;
+  return 0;
+}
+
+// per-tile squared errors of every batch of one epoch
+int ensure_loss(Mlp* m, int count) {
+  if (count > m->loss_cap) {
+    if (m->lossp) MCHK(hipFree(m->lossp));
+    MCHK(hipMalloc((void**)&m->lossp, (size_t)count * sizeof(double)));
+    m->loss_cap = count;
   }
   return 0;
 }
@@ -634,17 +657,21 @@ int cc_mlp_grad(void* mlp, const float* X, const float* T, int N, double* loss, 
   Mlp* m = (Mlp*)mlp;
   if (!m || !X || !T || N < 1) return mfail(CC_EINVAL, "bad arguments");
   if (ensure(m, N, N)) return CC_EHIP;
-  const int nchunk = (N + kChunk - 1) / kChunk;
-  if (ensure_part(m, nchunk)) return CC_EHIP;
+  const Chunks ck = chunks_for(N);
+  const int nw = (N + 15) / 16;
+  if (ensure_part(m, ck.n) || ensure_loss(m, nw)) return CC_EHIP;
   MCHK(hipMemcpyAsync(m->X, X, (size_t)N * 16 * sizeof(float), hipMemcpyHostToDevice, m->s));
   MCHK(hipMemcpyAsync(m->T, T, (size_t)N * 16 * sizeof(float), hipMemcpyHostToDevice, m->s));
-  double sse = 0.0;
-  if (batch_grad(m, nullptr, N, nchunk, &sse)) return CC_EHIP;
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNP + 255) / 256), dim3(256), 0, m->s, m->part, nchunk,
-                     m->grad, m->P, m->m, m->v, m->wb, 0, 0.f, 1.f, 1.f);
+  if (batch_grad(m, nullptr, N, ck, m->lossp)) return CC_EHIP;
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNP + kRedParams - 1) / kRedParams), dim3(256), 0, m->s,
+                     m->part, ck.n, m->grad, m->P, m->m, m->v, m->wb, 0, 0.f, 1.f, 1.f);
   MCHK(hipGetLastError());
+  std::vector<double> lp(nw);
+  MCHK(hipMemcpyAsync(lp.data(), m->lossp, nw * sizeof(double), hipMemcpyDeviceToHost, m->s));
   if (grad) MCHK(hipMemcpyAsync(grad, m->grad, kNP * sizeof(float), hipMemcpyDeviceToHost, m->s));
   MCHK(hipStreamSynchronize(m->s));
+  double sse = 0.0;
+  for (double x : lp) sse += x;
   if (loss) *loss = sse / (9.0 * N);
   return 0;
 }
@@ -655,31 +682,41 @@ int cc_mlp_train(void* mlp, const float* X, const float* T, int N, int epochs, i
   if (!m || !X || !T || N < 16 || epochs < 1 || batch < 16) return mfail(CC_EINVAL, "bad arguments");
   batch = std::min(batch, N);
   if (ensure(m, batch, N)) return CC_EHIP;
-  const int nchunk = (batch + kChunk - 1) / kChunk;
-  if (ensure_part(m, nchunk)) return CC_EHIP;
+  const Chunks ck = chunks_for(batch);
+  const int nw = (batch + 15) / 16, nbatches = N / batch;
+  if (ensure_part(m, ck.n) || ensure_loss(m, nbatches * nw)) return CC_EHIP;
+  std::vector<double> lp(loss_hist ? (size_t)nbatches * nw : 0);
   MCHK(hipMemcpyAsync(m->X, X, (size_t)N * 16 * sizeof(float), hipMemcpyHostToDevice, m->s));
   MCHK(hipMemcpyAsync(m->T, T, (size_t)N * 16 * sizeof(float), hipMemcpyHostToDevice, m->s));
   std::vector<int> perm(N);
   for (int i = 0; i < N; i++) perm[i] = i;
   std::mt19937 rng(seed);
   for (int ep = 0; ep < epochs; ep++) {
+    MCHK(hipStreamSynchronize(m->s));  // the previous epoch's copy of perm has been consumed
     std::shuffle(perm.begin(), perm.end(), rng);
     MCHK(hipMemcpyAsync(m->idx, perm.data(), (size_t)N * sizeof(int), hipMemcpyHostToDevice, m->s));
-    double ep_sse = 0.0;
-    long long ep_n = 0;
-    for (int b0 = 0; b0 + batch <= N; b0 += batch) {
-      double sse = 0.0;
-      if (batch_grad(m, m->idx + b0, batch, nchunk, loss_hist ? &sse : nullptr)) return CC_EHIP;
+    for (int bi = 0; bi < nbatches; bi++) {
+      if (batch_grad(m, m->idx + (size_t)bi * batch, batch, ck, m->lossp + (size_t)bi * nw))
+        return CC_EHIP;
       m->step++;
       const float c1 = 1.f - std::pow(0.9f, (float)m->step);
       const float c2 = 1.f - std::pow(0.999f, (float)m->step);
-      hipLaunchKernelGGL(k_mlp_reduce, dim3((kNP + 255) / 256), dim3(256), 0, m->s, m->part,
-                         nchunk, m->grad, m->P, m->m, m->v, m->wb, 1, lr, c1, c2);
-      ep_sse += sse;
-      ep_n += batch;
+      hipLaunchKernelGGL(k_mlp_reduce, dim3((kNP + kRedParams - 1) / kRedParams), dim3(256), 0,
+                         m->s, m->part, ck.n, m->grad, m->P, m->m, m->v, m->wb, 1, lr, c1, c2);
     }
     MCHK(hipGetLastError());
-    if (loss_hist) loss_hist[ep] = ep_sse / (9.0 * (double)ep_n);
+    if (loss_hist) {  // one read per epoch; batch sums, then the epoch, in order
+      MCHK(hipMemcpyAsync(lp.data(), m->lossp, lp.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, m->s));
+      MCHK(hipStreamSynchronize(m->s));
+      double ep_sse = 0.0;
+      for (int bi = 0; bi < nbatches; bi++) {
+        double t = 0.0;
+        for (int w = 0; w < nw; w++) t += lp[(size_t)bi * nw + w];
+        ep_sse += t;
+      }
+      loss_hist[ep] = ep_sse / (9.0 * (double)nbatches * batch);
+    }
   }
   MCHK(hipStreamSynchronize(m->s));
   return 0;
