@@ -62,8 +62,9 @@ def run_gpu(var, precision, steps, warmup, profile, keep=False):
     kms, kn = ch.profile(0) if profile else (0.0, 0)
     tau = ch.stats()[0]
     final = ch.download()[0] if keep else None
+    tiling = ch.tiling()
     ch.close()
-    return (t1 - t0), kms, kn, tau, final
+    return (t1 - t0), kms, kn, tau, final, tiling
 
 
 def run_ensemble(steps, warmup):
@@ -196,8 +197,8 @@ def main():
     var = formats.laminar_ic(n1, n2, n3)
 
     log(f"[bench] {args.precision}: warmup {args.warmup}, timed {args.steps} steps")
-    t, kms, kn, tau, v64 = run_gpu(var, args.precision, args.steps, args.warmup, profile=True,
-                                   keep=not args.no_mixed)
+    t, kms, kn, tau, v64, tiling = run_gpu(var, args.precision, args.steps, args.warmup,
+                                           profile=True, keep=not args.no_mixed)
     value = nodes * 2 * args.steps / t
     ms_step = 1000.0 * t / args.steps
     avg_launch_s = (kms / kn) / 1000.0 if kn else float("nan")
@@ -207,8 +208,8 @@ def main():
 
     mixed = None
     if not args.no_mixed and args.precision != "mixed":
-        tm, kmm, knm, taum, vmix = run_gpu(var, "mixed", args.steps, args.warmup, profile=True,
-                                           keep=True)
+        tm, kmm, knm, taum, vmix, tilm = run_gpu(var, "mixed", args.steps, args.warmup,
+                                                 profile=True, keep=True)
         vm = nodes * 2 * args.steps / tm
         names = ("rho", "u", "v", "w", "E", "T", "p")
         err = {nm: float(np.linalg.norm(vmix[c] - v64[c]) / np.linalg.norm(v64[c]))
@@ -218,6 +219,7 @@ def main():
         mixed = {"value": vm, "ms_per_step": 1000.0 * tm / args.steps,
                  "ratio_vs_fp64": vm / value,
                  "stage_kernel_ms": (kmm / knm) if knm else None,
+                 "tiling": tilm,
                  "rel_l2_vs_fp64_after_steps": err,
                  "mean_u_profile_rel_err": float(np.abs(ubm - ub64).max() / np.abs(ub64).max()),
                  "tau_wall": [float(x) for x in taum], "tau_wall_fp64": [float(x) for x in tau]}
@@ -259,6 +261,7 @@ def main():
             "workload": f"channel DNS time step, {n1}x{n3}x{n2} (reference nP=[{n1},{n2},{n3}]), "
                         "Ma 1.5, Re 3000, RK2, hybrid central/QUICK, seeded perturbed laminar start",
             "nP": list(NP), "dt": DT, "istat": ISTAT, "precision": args.precision,
+            "tiling": tiling,
             "mixed": mixed,
             "ensemble": ens,
         },

@@ -316,7 +316,8 @@ int cc_create(const cc_params* p, void** handle) {
     a.blow = c->blow;
     a.n1 = c->n1;
     a.n2 = c->n2;
-    set_tiling(a, c->variant, c->kchunk);
+    c->variant = choose_tiling(a, p->precision == CC_MIXED);
+    c->kchunk = a.kchunk;
   }
   if (hipMemset(c->force, 0, (2 + n3) * sizeof(double)) != hipSuccess ||
       hipMemset(c->stats, 0, (kNStat * n3 + 2) * sizeof(double)) != hipSuccess) {
@@ -490,6 +491,18 @@ int cc_profile(void* handle, int mode, double* out) {
     out[0] = c->prof_ms;
     out[1] = (double)c->prof_n;
   }
+  return 0;
+}
+
+int cc_tiling(void* handle, int* out) {
+  Chan* c = (Chan*)handle;
+  if (!c || !out) return fail(CC_EINVAL, "null argument");
+  int tx, ty;
+  tile_of(c->variant, tx, ty);
+  out[0] = tx;
+  out[1] = ty;
+  out[2] = c->inst.kchunk;
+  out[3] = c->inst.ntx * c->inst.nty * c->inst.nz;
   return 0;
 }
 

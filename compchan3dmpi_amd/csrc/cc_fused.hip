@@ -887,6 +887,65 @@ void set_tiling(Inst& a, int variant, int kchunk) {
 }
 
 template <typename T, int TX, int TY>
+static int fused_bpc() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_fused<T, TX, TY>, TX * TY, 0) !=
+      hipSuccess)
+    n = 1;
+  return n > 0 ? n : 1;
+}
+
+// resident blocks per CU of the one-instance stage kernel
+static int fused_blocks_per_cu(int mixed, int variant) {
+  if (!mixed) return fused_bpc<double, 32, 16>();
+  switch (variant) {
+    case 1: return fused_bpc<float, 32, 8>();
+    case 2: return fused_bpc<float, 16, 16>();
+    case 3: return fused_bpc<float, 64, 8>();
+    default: return fused_bpc<float, 32, 16>();
+  }
+}
+
+// Tiling of one instance: the tile shape (FP32 only) and the planes per
+// block that minimise the launch's critical path.  A block marches
+// kchunk + 3 planes (two below its chunk, one above) at a near-constant time
+// per plane, whatever the grid, and the launch runs in ceil(blocks / resident
+// slots) rounds; so the cost is rounds x (kchunk + 3).  Small grids get thin
+// chunks and many blocks, large ones a single round of long columns.  Ties
+// go to the earlier shape in the preference order and, within a shape, to
+// the longer chunk (fewer halo planes re-read).  Returns the variant.
+int choose_tiling(Inst& a, int mixed) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      ncu < 1)
+    ncu = 256;
+  const int nint = a.g.n3 - 2;
+  static const int kOrder[4] = {1, 0, 2, 3};  // FP32: 32x8 first (two blocks per CU)
+  long long best = -1;
+  int bv = 0, bk = 1;
+  for (int i = 0; i < (mixed ? 4 : 1); i++) {
+    const int v = mixed ? kOrder[i] : 0;
+    int tx, ty;
+    tile_of(v, tx, ty);
+    const long long tiles =
+        (long long)((a.g.P1 + tx - 1) / tx) * (long long)((a.g.P2 + ty - 1) / ty);
+    const long long slots = (long long)ncu * fused_blocks_per_cu(mixed, v);
+    for (int kc = 1; kc <= nint; kc++) {
+      const long long nz = (nint + kc - 1) / kc;
+      const long long cost = ((tiles * nz + slots - 1) / slots) * (kc + 3);
+      if (best < 0 || cost < best || (cost == best && v == bv && kc > bk)) {
+        best = cost;
+        bv = v;
+        bk = kc;
+      }
+    }
+  }
+  set_tiling(a, bv, bk);
+  return bv;
+}
+
+template <typename T, int TX, int TY>
 static void launch_one(const Inst& a, int stage, int step, hipStream_t s) {
   const unsigned nblk = (unsigned)(a.ntx * a.nty * a.nz);
   hipLaunchKernelGGL((k_stage_fused<T, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);

@@ -69,6 +69,9 @@ struct Inst {
 // tile variants: 0 = 32x16 (the only FP64 one), 1 = 32x8, 2 = 16x16, 3 = 64x8
 void tile_of(int variant, int& tx, int& ty);
 void set_tiling(Inst& a, int variant, int kchunk);
+// one instance: the tile shape and planes per block of least launch time on
+// the current device; returns the variant
+int choose_tiling(Inst& a, int mixed);
 void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int step,
                         hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                         bool wall = true);

@@ -98,6 +98,14 @@ int cc_blown(void* handle, int* ijk);
  * ms, out[1] = number of timed launches since the reset (out may be NULL). */
 int cc_profile(void* handle, int mode, double* out);
 
+/* Tiling the fused stage kernel uses on this handle (no counterpart in the
+ * reference, which decomposes the domain over MPI ranks at
+ * main_v1.9.f90:153-200): out[0], out[1] = tile nodes in x and y, out[2] =
+ * planes per block, out[3] = blocks per launch.  Chosen at cc_create to
+ * minimise the launch's critical path on the current device; ignored by
+ * fp64-strict. */
+int cc_tiling(void* handle, int* out);
+
 void cc_destroy(void* handle);
 
 /* ---- batched ensembles (no counterpart in the reference, whose ensembles are
