@@ -35,9 +35,8 @@ constexpr int kW1 = 0, kB1 = 1024, kW2 = 1088, kB2 = 5184, kW3 = 5248, kB3 = 934
               kB4 = 10432, kNP = 10448;
 constexpr int kNin = 11, kNout = 9;
 
-__device__ __forceinline__ short f2bf(float x) {  // round to nearest even
-  const unsigned u = __float_as_uint(x);
-  return (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+__device__ __forceinline__ short f2bf(float x) {  // v_cvt_pk_bf16_f32: nearest even
+  return __builtin_bit_cast(short, (__bf16)x);
 }
 __device__ __forceinline__ float bf2f(short b) {
   return __uint_as_float(((unsigned)(unsigned short)b) << 16);
@@ -50,33 +49,85 @@ __device__ __forceinline__ bf16x4 to_bf(f32x4 v) {
   r[3] = f2bf(v[3]);
   return r;
 }
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678f)); }
+// The standard normal CDF Phi(x) = (1 + erf(x/sqrt 2))/2 and e = exp(-x^2/2),
+// from Abramowitz & Stegun 7.1.26: for z >= 0,
+//   erfc z = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2),  t = 1/(1 + p z),
+// with an absolute error of at most 1.5e-7 (far below the BF16 rounding the
+// activations get next).  With z = |x|/sqrt 2, exp(-z^2) = e, and Phi takes
+// erfc directly on the negative side, so 1 + erf never cancels.
+__device__ __forceinline__ float norm_cdf(float x, float& e) {
+  // t = 1/(1 + p z) with p z = (0.3275911/sqrt 2)|x|; the polynomial carries
+  // the factor 1/2 of erfc/2 (coefficients a_i/2); e = 2^(-x^2 log2(e)/2)
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.23164190f, fabsf(x), 1.0f));
+  float p = fmaf(0.5307027145f, t, -0.7265760135f);
+  p = fmaf(p, t, 0.7107068705f);
+  p = fmaf(p, t, -0.142248368f);
+  p = fmaf(p, t, 0.127414796f);
+  e = __builtin_amdgcn_exp2f((x * x) * -0.72134752f);
+  const float h = (p * t) * e;  // erfc(|x|/sqrt 2)/2
+  return x < 0.f ? h : 1.0f - h;
+}
+// GELU(x) = x Phi(x); GELU'(x) = Phi(x) + x phi(x), phi(x) = e/sqrt(2 pi)
+__device__ __forceinline__ float gelu(float x) {
+  float e;
+  return x * norm_cdf(x, e);
+}
 __device__ __forceinline__ float dgelu(float x) {
-  return 0.5f * (1.0f + erff(x * 0.70710678f)) + x * 0.39894228f * expf(-0.5f * x * x);
+  float e;
+  const float c = norm_cdf(x, e);
+  return fmaf(x * 0.39894228f, e, c);
 }
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// A operand of W^T for a (output block ob, input block kb): A[m][k] =
-// W[kb*16 + k][ob*16 + m], lane holds m = s, k = 4q+j.  ld = row length of W.
-__device__ __forceinline__ bf16x4 wT(const short* W, int ld, int kb, int ob, int s, int q) {
-  bf16x4 a;
-  for (int j = 0; j < 4; j++) a[j] = W[(kb * 16 + 4 * q + j) * ld + ob * 16 + s];
-  return a;
+// BF16 weights stored as MFMA A operands: operand o, lane l = 16q + s holds
+// its 4 values at wpk[(o*64 + l)*4].  Lane (s, q), element j of
+//   a forward operand (W^T block (kb, ob)): W[kb*16 + 4q + j][ob*16 + s]
+//     W1 ob -> ob; W2 (kb, ob) -> 4 + 4kb + ob; W3 -> 20 + 4kb + ob; W4 kb -> 36 + kb;
+//   a backward operand (W block (ib, kb)): W[ib*16 + s][kb*16 + 4q + j]
+//     W4 ib -> 40 + ib; W3 (ib, kb) -> 44 + 4ib + kb; W2 -> 60 + 4ib + kb.
+// Each weight sits once in the forward operands and, W2..W4, once in the
+// backward ones; together they fill all 76 operands.
+constexpr int kNOps = 76;
+constexpr int kOpW1 = 0, kOpW2 = 4, kOpW3 = 20, kOpW4 = 36;  // forward
+constexpr int kOpB4 = 40, kOpB3 = 44, kOpB2 = 60;            // backward
+
+__device__ __forceinline__ bf16x4 wop(const short* __restrict__ wpk, int op, int lane) {
+  return *(const bf16x4*)(wpk + ((size_t)op * 64 + lane) * 4);
 }
-// A operand of W for the backward pass: A[m][k] = W[ib*16 + m][kb*16 + k].
-__device__ __forceinline__ bf16x4 wN(const short* W, int ld, int ib, int kb, int s, int q) {
-  bf16x4 a;
-  for (int j = 0; j < 4; j++) a[j] = W[(ib * 16 + s) * ld + kb * 16 + 4 * q + j];
-  return a;
+
+// store the BF16 value of parameter p at its operand positions (biases: none)
+__device__ __forceinline__ void pack_weight(short* wpk, int p, short v) {
+  int layer, r, c;
+  if (p < kB1) {
+    layer = 0, r = (p - kW1) / 64, c = (p - kW1) % 64;
+  } else if (p >= kW2 && p < kB2) {
+    layer = 1, r = (p - kW2) / 64, c = (p - kW2) % 64;
+  } else if (p >= kW3 && p < kB3) {
+    layer = 2, r = (p - kW3) / 64, c = (p - kW3) % 64;
+  } else if (p >= kW4 && p < kB4) {
+    layer = 3, r = (p - kW4) / 16, c = (p - kW4) % 16;
+  } else {
+    return;
+  }
+  const int kb = r / 16, ob = c / 16;
+  const int of = layer == 0 ? kOpW1 + ob
+                 : layer == 1 ? kOpW2 + 4 * kb + ob
+                 : layer == 2 ? kOpW3 + 4 * kb + ob
+                              : kOpW4 + kb;
+  wpk[((size_t)of * 64 + ((r % 16) / 4) * 16 + (c % 16)) * 4 + (r % 4)] = v;
+  if (layer == 0) return;
+  const int ib = r / 16, kc = c / 16;
+  const int obk = layer == 3 ? kOpB4 + ib : layer == 2 ? kOpB3 + 4 * ib + kc : kOpB2 + 4 * ib + kc;
+  wpk[((size_t)obk * 64 + ((c % 16) / 4) * 16 + (r % 16)) * 4 + (c % 4)] = v;
 }
 
 // Forward pass; Z (optional) receives the pre-activations Z1|Z2|Z3 as
 // [N][192]; idx (optional) gathers the samples.
 __global__ void __launch_bounds__(256) k_mlp_fwd(const float* __restrict__ X,
                                                  const int* __restrict__ idx, int N,
-                                                 const short* __restrict__ wb,
+                                                 const short* __restrict__ wpk,
                                                  const float* __restrict__ P,
                                                  float* __restrict__ Y, float* __restrict__ Z) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -90,28 +141,35 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(const float* __restrict__ X,
   if (valid) x = *(const f32x4*)(X + (size_t)src * 16 + 4 * q);
   const bf16x4 b0 = to_bf(x);
   f32x4 h[4], g[4];
+#pragma unroll
   for (int ob = 0; ob < 4; ob++) {
-    f32x4 z = mfma(wT(wb + kW1, 64, 0, ob, s, q), b0, f32x4{0.f, 0.f, 0.f, 0.f});
-    for (int j = 0; j < 4; j++) z[j] += P[kB1 + ob * 16 + 4 * q + j];
+    // the bias seeds the accumulator: lane (s, q) holds features ob*16 + 4q + j
+    f32x4 z = mfma(wop(wpk, kOpW1 + ob, lane), b0, *(const f32x4*)(P + kB1 + ob * 16 + 4 * q));
     if (Z && valid) *(f32x4*)(Z + (size_t)n * 192 + ob * 16 + 4 * q) = z;
+#pragma unroll
     for (int j = 0; j < 4; j++) h[ob][j] = gelu(z[j]);
   }
+#pragma unroll
   for (int L = 0; L < 2; L++) {
-    const int wo = L == 0 ? kW2 : kW3, bo = L == 0 ? kB2 : kB3;
+    const int wo = L == 0 ? kOpW2 : kOpW3, bo = L == 0 ? kB2 : kB3;
     bf16x4 hb[4];
+#pragma unroll
     for (int kb = 0; kb < 4; kb++) hb[kb] = to_bf(h[kb]);
+#pragma unroll
     for (int ob = 0; ob < 4; ob++) {
-      f32x4 z = {0.f, 0.f, 0.f, 0.f};
-      for (int kb = 0; kb < 4; kb++) z = mfma(wT(wb + wo, 64, kb, ob, s, q), hb[kb], z);
-      for (int j = 0; j < 4; j++) z[j] += P[bo + ob * 16 + 4 * q + j];
+      f32x4 z = *(const f32x4*)(P + bo + ob * 16 + 4 * q);
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) z = mfma(wop(wpk, wo + 4 * kb + ob, lane), hb[kb], z);
       if (Z && valid) *(f32x4*)(Z + (size_t)n * 192 + (L + 1) * 64 + ob * 16 + 4 * q) = z;
+#pragma unroll
       for (int j = 0; j < 4; j++) g[ob][j] = gelu(z[j]);
     }
+#pragma unroll
     for (int ob = 0; ob < 4; ob++) h[ob] = g[ob];
   }
-  f32x4 y = {0.f, 0.f, 0.f, 0.f};
-  for (int kb = 0; kb < 4; kb++) y = mfma(wT(wb + kW4, 16, kb, 0, s, q), to_bf(h[kb]), y);
-  for (int j = 0; j < 4; j++) y[j] += P[kB4 + 4 * q + j];
+  f32x4 y = *(const f32x4*)(P + kB4 + 4 * q);
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) y = mfma(wop(wpk, kOpW4 + kb, lane), to_bf(h[kb]), y);
   if (valid) *(f32x4*)(Y + (size_t)n * 16 + 4 * q) = y;
 }
 
@@ -121,7 +179,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(const float* __restrict__ X,
 // squared error of the wave's samples.
 __global__ void __launch_bounds__(256) k_mlp_bwd(const float* __restrict__ T,
                                                  const int* __restrict__ idx, int N,
-                                                 const short* __restrict__ wb,
+                                                 const short* __restrict__ wpk,
                                                  const float* __restrict__ Y,
                                                  const float* __restrict__ Z,
                                                  float* __restrict__ dZ,
@@ -140,6 +198,7 @@ __global__ void __launch_bounds__(256) k_mlp_bwd(const float* __restrict__ T,
   if (valid) {
     const f32x4 y = *(const f32x4*)(Y + (size_t)n * 16 + 4 * q);
     const f32x4 t = *(const f32x4*)(T + (size_t)src * 16 + 4 * q);
+#pragma unroll
     for (int j = 0; j < 4; j++)
       if (4 * q + j < kNout) {
         const float e = y[j] - t[j];
@@ -153,26 +212,34 @@ __global__ void __launch_bounds__(256) k_mlp_bwd(const float* __restrict__ T,
   // dA3 = W4 dZ4 (one K block of 16 outputs)
   f32x4 d[4];
   const bf16x4 b4 = to_bf(d4);
+#pragma unroll
   for (int ib = 0; ib < 4; ib++) {
-    f32x4 a = mfma(wN(wb + kW4, 16, ib, 0, s, q), b4, f32x4{0.f, 0.f, 0.f, 0.f});
+    f32x4 a = mfma(wop(wpk, kOpB4 + ib, lane), b4, f32x4{0.f, 0.f, 0.f, 0.f});
     f32x4 z = {0.f, 0.f, 0.f, 0.f};
     if (valid) z = *(const f32x4*)(Z + (size_t)n * 192 + 128 + ib * 16 + 4 * q);
+#pragma unroll
     for (int j = 0; j < 4; j++) d[ib][j] = a[j] * dgelu(z[j]);
     if (valid) *(f32x4*)(dZ + (size_t)n * 208 + 128 + ib * 16 + 4 * q) = d[ib];
   }
+#pragma unroll
   for (int L = 1; L >= 0; L--) {  // layer 3 weights (L=1) then layer 2 (L=0)
-    const int wo = L == 1 ? kW3 : kW2;
+    const int wo = L == 1 ? kOpB3 : kOpB2;
     bf16x4 db[4];
+#pragma unroll
     for (int kb = 0; kb < 4; kb++) db[kb] = to_bf(d[kb]);
     f32x4 e[4];
+#pragma unroll
     for (int ib = 0; ib < 4; ib++) {
       f32x4 a = {0.f, 0.f, 0.f, 0.f};
-      for (int kb = 0; kb < 4; kb++) a = mfma(wN(wb + wo, 64, ib, kb, s, q), db[kb], a);
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) a = mfma(wop(wpk, wo + 4 * ib + kb, lane), db[kb], a);
       f32x4 z = {0.f, 0.f, 0.f, 0.f};
       if (valid) z = *(const f32x4*)(Z + (size_t)n * 192 + L * 64 + ib * 16 + 4 * q);
+#pragma unroll
       for (int j = 0; j < 4; j++) e[ib][j] = a[j] * dgelu(z[j]);
       if (valid) *(f32x4*)(dZ + (size_t)n * 208 + L * 64 + ib * 16 + 4 * q) = e[ib];
     }
+#pragma unroll
     for (int ib = 0; ib < 4; ib++) d[ib] = e[ib];
   }
 }
@@ -199,6 +266,7 @@ __global__ void __launch_bounds__(64) k_mlp_dw(const float* __restrict__ X,
   float bsum = 0.f;
   for (int s0 = c0; s0 < c1; s0 += 16) {
     bf16x4 a, b;
+#pragma unroll
     for (int j = 0; j < 4; j++) {
       const int n = s0 + 4 * q + j;
       float av = 0.f, bv = 0.f;
@@ -245,7 +313,7 @@ __global__ void __launch_bounds__(256) k_mlp_reduce(const float* __restrict__ pa
                                                     float* __restrict__ grad,
                                                     float* __restrict__ P, float* __restrict__ m,
                                                     float* __restrict__ v,
-                                                    short* __restrict__ wb, int adam, float lr,
+                                                    short* __restrict__ wpk, int adam, float lr,
                                                     float c1, float c2) {
   __shared__ float sl[4][kRedParams];
   const int t = threadIdx.x & (kRedParams - 1), slice = threadIdx.x / kRedParams;
@@ -270,12 +338,12 @@ __global__ void __launch_bounds__(256) k_mlp_reduce(const float* __restrict__ pa
   float w = P[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
   if (frozen(p)) w = 0.f;
   P[p] = w;
-  wb[p] = f2bf(w);
+  pack_weight(wpk, p, f2bf(w));
 }
 
-__global__ void k_mlp_tobf(const float* __restrict__ P, short* __restrict__ wb) {
+__global__ void k_mlp_tobf(const float* __restrict__ P, short* __restrict__ wpk) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < kNP) wb[p] = f2bf(P[p]);
+  if (p < kNP) pack_weight(wpk, p, f2bf(P[p]));
 }
 
 // ---- filtered DNS samples ---------------------------------------------------
@@ -472,7 +540,7 @@ int mfail(int code, const std::string& msg) {
 
 struct Mlp {
   float *P = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *part = nullptr;
-  short* wb = nullptr;
+  short* wpk = nullptr;  // BF16 weights as MFMA operands (kNOps x 64 lanes x 4)
   float *X = nullptr, *T = nullptr, *Y = nullptr, *Z = nullptr, *dZ = nullptr;
   int* idx = nullptr;
   double* lossp = nullptr;
@@ -523,9 +591,9 @@ Chunks chunks_for(int N) {
 // thousand samples then still spreads over every CU.
 int batch_grad(Mlp* m, const int* idx, int nb, Chunks ck, double* lossp) {
   const int nw = (nb + 15) / 16;
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(nw), dim3(64), 0, m->s, m->X, idx, nb, m->wb, m->P, m->Y,
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(nw), dim3(64), 0, m->s, m->X, idx, nb, m->wpk, m->P, m->Y,
                      m->Z);
-  hipLaunchKernelGGL(k_mlp_bwd, dim3(nw), dim3(64), 0, m->s, m->T, idx, nb, m->wb, m->Y, m->Z,
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(nw), dim3(64), 0, m->s, m->T, idx, nb, m->wpk, m->Y, m->Z,
                      m->dZ, lossp);
   hipLaunchKernelGGL(k_mlp_dw, dim3(40, ck.n), dim3(64), 0, m->s, m->X, idx, nb, m->Z, m->dZ,
                      ck.len, m->part);
@@ -567,7 +635,7 @@ void closure_stage(void* mlp, const Inst& a, const float* norm, float* X, float*
   hipLaunchKernelGGL(k_les_features, dim3((unsigned)((NI + B - 1) / B)), dim3(B), 0, s, a.g, a.ph,
                      Uin, a.invJac, a.uw, a.vw, norm, X);
   hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((NI + 63) / 64)), dim3(256), 0, s, X,
-                     (const int*)nullptr, (int)NI, m->wb, m->P, Y, (float*)nullptr);
+                     (const int*)nullptr, (int)NI, m->wpk, m->P, Y, (float*)nullptr);
   hipLaunchKernelGGL(k_les_apply, dim3((unsigned)((NI + B - 1) / B)), dim3(B), 0, s, a.g, a.ph,
                      Uin, a.invJac, Y, norm, stage == 1 ? 0.5 : 1.0, Uout);
 }
@@ -609,7 +677,7 @@ int cc_mlp_create(unsigned seed, void** out) {
       hipMalloc((void**)&m->m, kNP * sizeof(float)) != hipSuccess ||
       hipMalloc((void**)&m->v, kNP * sizeof(float)) != hipSuccess ||
       hipMalloc((void**)&m->grad, kNP * sizeof(float)) != hipSuccess ||
-      hipMalloc((void**)&m->wb, kNP * sizeof(short)) != hipSuccess) {
+      hipMalloc((void**)&m->wpk, (size_t)kNOps * 64 * 4 * sizeof(short)) != hipSuccess) {
     cc_mlp_destroy(m);
     return mfail(CC_ENOMEM, "mlp allocation failed");
   }
@@ -628,7 +696,7 @@ int cc_mlp_set_params(void* mlp, const float* in) {
   MCHK(hipMemset(m->m, 0, kNP * sizeof(float)));
   MCHK(hipMemset(m->v, 0, kNP * sizeof(float)));
   m->step = 0;
-  hipLaunchKernelGGL(k_mlp_tobf, dim3((kNP + 255) / 256), dim3(256), 0, m->s, m->P, m->wb);
+  hipLaunchKernelGGL(k_mlp_tobf, dim3((kNP + 255) / 256), dim3(256), 0, m->s, m->P, m->wpk);
   MCHK(hipStreamSynchronize(m->s));
   return 0;
 }
@@ -646,7 +714,7 @@ int cc_mlp_forward(void* mlp, const float* X, int N, float* Y) {
   if (ensure(m, N, N)) return CC_EHIP;
   MCHK(hipMemcpyAsync(m->X, X, (size_t)N * 16 * sizeof(float), hipMemcpyHostToDevice, m->s));
   hipLaunchKernelGGL(k_mlp_fwd, dim3((N + 63) / 64), dim3(256), 0, m->s, m->X, (const int*)nullptr,
-                     N, m->wb, m->P, m->Y, (float*)nullptr);
+                     N, m->wpk, m->P, m->Y, (float*)nullptr);
   MCHK(hipGetLastError());
   MCHK(hipMemcpyAsync(Y, m->Y, (size_t)N * 16 * sizeof(float), hipMemcpyDeviceToHost, m->s));
   MCHK(hipStreamSynchronize(m->s));
@@ -664,7 +732,7 @@ int cc_mlp_grad(void* mlp, const float* X, const float* T, int N, double* loss, 
   MCHK(hipMemcpyAsync(m->T, T, (size_t)N * 16 * sizeof(float), hipMemcpyHostToDevice, m->s));
   if (batch_grad(m, nullptr, N, ck, m->lossp)) return CC_EHIP;
   hipLaunchKernelGGL(k_mlp_reduce, dim3((kNP + kRedParams - 1) / kRedParams), dim3(256), 0, m->s,
-                     m->part, ck.n, m->grad, m->P, m->m, m->v, m->wb, 0, 0.f, 1.f, 1.f);
+                     m->part, ck.n, m->grad, m->P, m->m, m->v, m->wpk, 0, 0.f, 1.f, 1.f);
   MCHK(hipGetLastError());
   std::vector<double> lp(nw);
   MCHK(hipMemcpyAsync(lp.data(), m->lossp, nw * sizeof(double), hipMemcpyDeviceToHost, m->s));
@@ -702,7 +770,7 @@ int cc_mlp_train(void* mlp, const float* X, const float* T, int N, int epochs, i
       const float c1 = 1.f - std::pow(0.9f, (float)m->step);
       const float c2 = 1.f - std::pow(0.999f, (float)m->step);
       hipLaunchKernelGGL(k_mlp_reduce, dim3((kNP + kRedParams - 1) / kRedParams), dim3(256), 0,
-                         m->s, m->part, ck.n, m->grad, m->P, m->m, m->v, m->wb, 1, lr, c1, c2);
+                         m->s, m->part, ck.n, m->grad, m->P, m->m, m->v, m->wpk, 1, lr, c1, c2);
     }
     MCHK(hipGetLastError());
     if (loss_hist) {  // one read per epoch; batch sums, then the epoch, in order
@@ -727,7 +795,7 @@ void cc_mlp_destroy(void* mlp) {
   if (!m) return;
   for (float* b : {m->P, m->m, m->v, m->grad, m->part, m->X, m->T, m->Y, m->Z, m->dZ})
     if (b) (void)hipFree(b);
-  if (m->wb) (void)hipFree(m->wb);
+  if (m->wpk) (void)hipFree(m->wpk);
   if (m->idx) (void)hipFree(m->idx);
   if (m->lossp) (void)hipFree(m->lossp);
   if (m->s) (void)hipStreamDestroy(m->s);

@@ -58,6 +58,49 @@ def test_gradient_matches_reference():
         assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-12, blk
 
 
+def identity_params():
+    d = {k: np.zeros(v[1:]) for k, v in mlp_ref.OFF.items()}
+    for i in range(11):
+        d["W1"][i, i] = 1.0
+    for i in range(64):
+        d["W2"][i, i] = 1.0
+        d["W3"][i, i] = 1.0
+    for i in range(9):
+        d["W4"][i, i] = 1.0
+    return mlp_ref.pack(d)
+
+
+def test_identity_network_pins_the_activation():
+    """With identity weights each output is three GELUs, each followed by the
+    BF16 rounding of the next MFMA operand, and every MFMA sum has a single
+    term.  The BF16-emulating reference (exact erf in FP64) then agrees with
+    the device wherever the device's FP32 GELU rounds to the same BF16 value,
+    which fails only within ~1e-7 of a rounding boundary.  A misplaced
+    coefficient of the activation or its derivative (errors ~1e-3) would
+    move a large share of the outputs and bias gradients."""
+    banner("identity network: GELU and GELU' against exact erf")
+    m = Mlp(seed=0)
+    P = identity_params()
+    m.params = P
+    X, T = rand_problem(8192, 5)
+    X[:, :11] *= 1.5
+    Y = m.forward(X)
+    Yr, _ = mlp_ref.forward(P, X, emulate_bf16=True)
+    d = np.abs(Y[:, :9] - Yr[:, :9])
+    frac = float(np.mean(d <= 2e-6))
+    print(f"  outputs within 2e-6: {100 * frac:.3f} %, max |diff| {d.max():.2e}", flush=True)
+    assert frac >= 0.995 and d.max() < 2e-2
+    _, g = m.grad(X, T)
+    _, gr = mlp_ref.loss_grad(P, X, T, emulate_bf16=True)
+    for blk in ("b1", "b2", "b3", "b4", "W2", "W3", "W4"):
+        o, *shape = mlp_ref.OFF[blk]
+        n = int(np.prod(shape))
+        a, b = g[o:o + n], gr[o:o + n]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"  {blk}: rel err {rel:.2e}", flush=True)
+        assert rel <= 1e-4, blk
+
+
 def test_training_fits_a_teacher_network():
     banner("training on the device: fit a fixed teacher network")
     teacher = Mlp(seed=11)
