@@ -229,10 +229,12 @@ class Channel:
 
     def tiling(self):
         """Tiling of the fused stage kernel: {"tile": (tx, ty), "kchunk": planes
-        per block, "blocks": blocks per launch}."""
-        out = np.zeros(4, dtype=np.int32)
+        per block, "blocks": blocks per launch, "cus": the device's CUs,
+        "blocks_per_cu": resident blocks per CU}."""
+        out = np.zeros(6, dtype=np.int32)
         _check(self.L, self.L.cc_tiling(self.h, out.ctypes.data_as(C.c_void_p)))
-        return {"tile": (int(out[0]), int(out[1])), "kchunk": int(out[2]), "blocks": int(out[3])}
+        return {"tile": (int(out[0]), int(out[1])), "kchunk": int(out[2]), "blocks": int(out[3]),
+                "cus": int(out[4]), "blocks_per_cu": int(out[5])}
 
     def filter_samples(self):
         """4x4x4 box-filtered features (N,16) and subgrid targets (N,16) of

@@ -66,6 +66,7 @@ struct Chan {
   float *cnorm = nullptr, *cX = nullptr, *cY = nullptr;
   int variant = 0;  // tile shape of the stage kernel
   int kchunk = 0;   // planes per block (0 = automatic)
+  int tile_ncu = 0, tile_bpc = 0;  // CUs and resident blocks per CU the tiling assumed
   // per-launch timing of the stage's main kernel (cc_profile)
   bool prof = false;
   std::vector<hipEvent_t> evpool;
@@ -316,7 +317,7 @@ int cc_create(const cc_params* p, void** handle) {
     a.blow = c->blow;
     a.n1 = c->n1;
     a.n2 = c->n2;
-    c->variant = choose_tiling(a, p->precision == CC_MIXED);
+    c->variant = choose_tiling(a, p->precision == CC_MIXED, &c->tile_ncu, &c->tile_bpc);
     c->kchunk = a.kchunk;
   }
   if (hipMemset(c->force, 0, (2 + n3) * sizeof(double)) != hipSuccess ||
@@ -503,6 +504,8 @@ int cc_tiling(void* handle, int* out) {
   out[1] = ty;
   out[2] = c->inst.kchunk;
   out[3] = c->inst.ntx * c->inst.nty * c->inst.nz;
+  out[4] = c->tile_ncu;
+  out[5] = c->tile_bpc;
   return 0;
 }
 

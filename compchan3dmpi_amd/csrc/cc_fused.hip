@@ -174,11 +174,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   // region index of the thread's own node and its up to two ring nodes
   const int r_own = (ty + 2) * RX + (tx + 2);
   const int gi_own = wrapP(i0 + tx, P1), gj_own = wrapP(j0 + ty, P2);
-  const long long o_own = (long long)gj_own * P1 + gi_own;
-  int r_ring[2];
-  long long o_ring[2];
-  int nring = 0;
-  for (int q = tid; q < NRING && nring < 2; q += NT) {
+  const int o_own = gj_own * P1 + gi_own;  // offset within a plane
+  // the thread's ring node: with NRING <= NT, threads tid < NRING own one
+  const bool has_ring = tid < NRING;
+  int r_ring = 0, o_ring = 0;
+  if (has_ring) {
+    const int q = tid;
     int rx, ry;
     if (q < 2 * RX) {  // rows 0,1
       ry = q / RX;
@@ -193,9 +194,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       const int c = m % 4;
       rx = c < 2 ? c : TX + c;
     }
-    r_ring[nring] = ry * RX + rx;
-    o_ring[nring] = (long long)wrapP(j0 + ry - 2, P2) * P1 + wrapP(i0 + rx - 2, P1);
-    nring++;
+    r_ring = ry * RX + rx;
+    o_ring = wrapP(j0 + ry - 2, P2) * P1 + wrapP(i0 + rx - 2, P1);
   }
   // ring-1 slot of a region index (x ring: rx in {1, TX+2}; y ring: ry in {1, TY+2})
   auto zring = [&](int rx, int ry) -> int {
@@ -204,6 +204,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       return 2 * TY + (ry == 1 ? 0 : TX) + (rx - 2);
     return -1;
   };
+  const int zr_ring = has_ring ? zring(r_ring % RX, r_ring / RX) : -1;
   // register windows along the own column
   double Uw[5][5];  // U at planes t-4..t (index 4 = t)
   T ww[5];          // w at t-4..t
@@ -232,7 +233,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 
   // primitives of a loaded node; walls take the boundary values
   // (main_v1.9.f90:624-643)
-  auto prim = [&](long long off, int t, T& u, T& v, T& w, T& Tt, T& p, T& mu,
+  auto prim = [&](int off, int t, T& u, T& v, T& w, T& Tt, T& p, T& mu,
                   const double (&Uo)[5]) {
     if (t == 0 || t == n3 - 1) {
       const long long wo = (t == 0 ? 0 : PL) + off;
@@ -276,9 +277,9 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     const long long n = (long long)t * PL;
 #pragma unroll
     for (int c = 0; c < 5; c++) Un1[c] = Uin[c * F + n + o_own];
-    if (nring > 0) {
+    if (has_ring) {
 #pragma unroll
-      for (int c = 0; c < 5; c++) Ur1[c] = Uin[c * F + n + o_ring[0]];
+      for (int c = 0; c < 5; c++) Ur1[c] = Uin[c * F + n + o_ring];
     }
   };
   if constexpr (kAhead) load_plane(tb);
@@ -329,18 +330,18 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       // z scheme of this plane (main_v1.9.f90:409-416)
       const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(a.jac[t]) > T(ph.Pez) * mu;
       zq = (zq << 1) | (qz ? 1u : 0u);
-      for (int q = 0; q < nring; q++) {
+      if (has_ring) {
         if constexpr (kAhead) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Uo[c] = Ur1[c];
         } else {
-          const long long n = (long long)t * PL + o_ring[q];
+          const long long n = (long long)t * PL + o_ring;
 #pragma unroll
           for (int c = 0; c < 5; c++) Uo[c] = Uin[c * F + n];
         }
-        prim(o_ring[q], t, u, v, w, Tt, p, mu, Uo);
-        const int r = r_ring[q];
-        const int zr = zring(r % RX, r / RX);
+        prim(o_ring, t, u, v, w, Tt, p, mu, Uo);
+        const int r = r_ring;
+        const int zr = zr_ring;
         if (zr >= 0) {
           srz[t % 3][0][zr] = u;
           srz[t % 3][1][zr] = v;
@@ -914,7 +915,7 @@ static int fused_blocks_per_cu(int mixed, int variant) {
 // chunks and many blocks, large ones a single round of long columns.  Ties
 // go to the earlier shape in the preference order and, within a shape, to
 // the longer chunk (fewer halo planes re-read).  Returns the variant.
-int choose_tiling(Inst& a, int mixed) {
+int choose_tiling(Inst& a, int mixed, int* ncu_out, int* bpc_out) {
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -942,6 +943,8 @@ int choose_tiling(Inst& a, int mixed) {
     }
   }
   set_tiling(a, bv, bk);
+  if (ncu_out) *ncu_out = ncu;
+  if (bpc_out) *bpc_out = fused_blocks_per_cu(mixed, bv);
   return bv;
 }
 

@@ -70,8 +70,9 @@ struct Inst {
 void tile_of(int variant, int& tx, int& ty);
 void set_tiling(Inst& a, int variant, int kchunk);
 // one instance: the tile shape and planes per block of least launch time on
-// the current device; returns the variant
-int choose_tiling(Inst& a, int mixed);
+// the current device; returns the variant, and the CU count and resident
+// blocks per CU it assumed (either pointer may be null)
+int choose_tiling(Inst& a, int mixed, int* ncu_out, int* bpc_out);
 void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int step,
                         hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                         bool wall = true);

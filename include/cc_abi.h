@@ -101,8 +101,9 @@ int cc_profile(void* handle, int mode, double* out);
 /* Tiling the fused stage kernel uses on this handle (no counterpart in the
  * reference, which decomposes the domain over MPI ranks at
  * main_v1.9.f90:153-200): out[0], out[1] = tile nodes in x and y, out[2] =
- * planes per block, out[3] = blocks per launch.  Chosen at cc_create to
- * minimise the launch's critical path on the current device; ignored by
+ * planes per block, out[3] = blocks per launch, out[4] = the device's CUs,
+ * out[5] = resident blocks per CU.  Chosen at cc_create to minimise
+ * ceil(blocks / (out[4] * out[5])) * (planes per block + 3); ignored by
  * fp64-strict. */
 int cc_tiling(void* handle, int* out);
 

@@ -1,0 +1,56 @@
+"""Launch tiling of the fused stage kernel (cc_tiling): the planes per block
+and, for FP32, the tile shape that minimise rounds x (planes + 3), where a
+round fills every resident slot (CUs x blocks per CU) once."""
+import pytest
+
+from compchan3dmpi_amd.channel import Channel
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(64, 32, 48), (96, 64, 48), (160, 96, 80), (256, 128, 192), (34, 128, 192),
+         (7, 9, 6), (37, 21, 11)]
+
+# (grid, precision) -> (tile, planes per block, blocks) on a 256-CU MI355X:
+# FP64 32x16 holds one block per CU, FP32 32x8 two.
+EXPECTED_256 = {
+    ((64, 32, 48), "fp64"): ((32, 16), 1, 184),      # 4 tiles x 46 chunks, one round
+    ((64, 32, 48), "mixed"): ((32, 8), 1, 368),
+    ((96, 64, 48), "fp64"): ((32, 16), 3, 192),      # 12 tiles x 16 chunks
+    ((160, 96, 80), "fp64"): ((32, 16), 10, 240),    # 30 tiles x 8 chunks
+    ((256, 128, 192), "fp64"): ((32, 16), 48, 256),  # 64 tiles x 4 chunks, one round
+    ((256, 128, 192), "mixed"): ((32, 8), 48, 512),
+    ((34, 128, 192), "fp64"): ((32, 16), 6, 256),    # an 8-way x slab of the above
+}
+
+
+def cdiv(a, b):
+    return -(-a // b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("grid", GRIDS)
+def test_tiling_is_the_model_optimum(grid, precision):
+    print(f"\n[case] tiling {grid} {precision}", flush=True)
+    ch = Channel(*grid, dt=1e-4, precision=precision)
+    try:
+        t = ch.tiling()
+    finally:
+        ch.close()
+    (tx, ty), kc = t["tile"], t["kchunk"]
+    P1, P2, nint = grid[0] - 2, grid[1] - 2, grid[2] - 2
+    tiles = cdiv(P1, tx) * cdiv(P2, ty)
+    assert t["blocks"] == tiles * cdiv(nint, kc)
+    if precision == "fp64":
+        assert (tx, ty) == (32, 16)
+    slots = t["cus"] * t["blocks_per_cu"]
+    assert slots >= 1
+
+    def cost(k):
+        return cdiv(tiles * cdiv(nint, k), slots) * (k + 3)
+
+    best = min(cost(k) for k in range(1, nint + 1))
+    assert cost(kc) == best
+    # ties go to the longer chunk
+    assert all(cost(k) > best for k in range(kc + 1, nint + 1))
+    if t["cus"] == 256 and (grid, precision) in EXPECTED_256:
+        assert ((tx, ty), kc, t["blocks"]) == EXPECTED_256[(grid, precision)], t
