@@ -133,7 +133,6 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   constexpr int NFX = (TX + 2) * TY;            // fD points (tile + x halo 1)
   constexpr int NGY = TX * (TY + 2);            // gD points (tile + y halo 1)
   constexpr int NFG = NFX > NGY ? NFX : NGY;
-  constexpr int NUB = (5 * NR > 4 * (NFX + NGY)) ? 5 * NR : 4 * (NFX + NGY);
   constexpr int NZR = 2 * TY + 2 * TX;          // x- and y- ring-1 points
   static_assert(NRING <= NT, "one ring node per thread");
   // first threads of the y-ring (gD) and x-ring (fD) point-flux passes
@@ -143,7 +142,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 
   __shared__ T sl[2][4][NR];    // u, v, w, T of two planes
   __shared__ T spm[2][2][NR];   // p, mu of two planes
-  __shared__ T sub[NUB];        // U(5) of plane t, then fD and gD of plane t-1
+  __shared__ T sub[5 * NR];     // U(5) of plane t (phase 2)
+  __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
 
   const Geom g = a.g;
@@ -423,7 +423,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
         }
       }
     }
-    __syncthreads();  // sub is reused for fD / gD below
+    // no barrier: phase 3 reads plane t-1's slots and the z-ring (complete
+    // since phase 1) and writes only fD / gD, which phase 2 does not touch
 
     // ---- phase 3: point fluxes of plane p = t-1 (main_v1.9.f90:457-563)
     const int p = t - 1;
@@ -502,7 +503,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     }
     if (need_fg) {
       // fD on the tile and its x ring-1 columns
-      T* fx = sub;  // [4][(TX+2)*TY], index (ty)*(TX+2) + (rx-1)
+      T* fx = sfg;  // [4][(TX+2)*TY], index (ty)*(TX+2) + (rx-1)
       {
         const int li = ty * (TX + 2) + (tx + 1);
 #pragma unroll
@@ -537,7 +538,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     }
     if (need_fg) {
       // gD on the tile and its y ring-1 rows
-      T* gy = sub + 4 * NFX;  // [4][TX*(TY+2)], index (ry-1)*TX + tx
+      T* gy = sfg + 4 * NFX;  // [4][TX*(TY+2)], index (ry-1)*TX + tx
       {
         const int li = (ty + 1) * TX + tx;
 #pragma unroll
@@ -572,8 +573,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       if (t + 1 <= tlast) load_plane(t + 1);
     }
     if (need_fg) {
-      const T* fx = sub;
-      const T* gy = sub + 4 * NFX;
+      const T* fx = sfg;
+      const T* gy = sfg + 4 * NFX;
       const int lf = ty * (TX + 2) + (tx + 1);
       const int lg = (ty + 1) * TX + tx;
 #pragma unroll
