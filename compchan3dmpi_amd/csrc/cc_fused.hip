@@ -705,15 +705,22 @@ __global__ void __launch_bounds__(TX* TY) k_stage_batched(const Inst* __restrict
 // threads on the domain edges also update the wall's periodic duplicates
 // (the reference's boundary loop covers them at their own coordinates) and
 // the four corners, whose interior columns the reference never refreshes.
-__device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, int ii, int jj,
-                                          int wall) {
+// One 64 x 4 block of wall nodes.  The block first recovers the primitives of
+// its patch plus a 1-node x/y halo at the two planes next to the wall, once per
+// column, into LDS; each node then reads the five columns it needs from there.
+constexpr int kWX = 64, kWY = 4, kWLX = kWX + 2, kWLY = kWY + 2;
+__device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, int wall) {
+  __shared__ double wq[7][2][kWLY][kWLX];  // r, u, v, w, T, p, mu
   const Geom& g = a.g;
   const Phys& ph = a.ph;
-  if (ii >= g.P1 || jj >= g.P2) return;
-  {
-    const int h = *a.halt;
-    if (h != 0 && h < step) return;
-  }
+  const int i0 = blockIdx.x * kWX, j0 = blockIdx.y * kWY;
+  if (i0 >= g.P1 || j0 >= g.P2) return;  // block-uniform (batches of smaller instances)
+  // The halt flag, the patch's state and the node's old wall pressure are
+  // loaded together (one memory latency, not three); the halt is tested once
+  // they are in flight.  A halt set during this launch holds this step, so
+  // the test is block-uniform.
+  const int h = *a.halt;
+  const int ii = i0 + threadIdx.x, jj = j0 + threadIdx.y;
   const double* __restrict__ U = stage == 1 ? a.UB : a.UA;
   double* __restrict__ pw = a.pw;
   double* __restrict__ rw = a.rw;
@@ -737,17 +744,64 @@ __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, in
   struct P {
     double r, u, v, w, T, p, mu;
   };
-  auto prim = [&](int i, int j, int k) {
-    const long long n = (long long)k * PL + (long long)j * g.P1 + i;
+  const bool valid = ii < g.P1 && jj < g.P2;
+  const double pold = valid ? pw[wall * PL + n2d] : 0.0;
+  {
+    // the patch's columns at planes k1, k2 (its four corners are not needed),
+    // up to NPER per thread, all loads issued before any is used
+    auto wrapm = [](int x, int P) {
+      x %= P;
+      return x < 0 ? x + P : x;
+    };
+    constexpr int NE = 2 * kWLX * kWLY, NT = kWX * kWY, NPER = (NE + NT - 1) / NT;
+    const int tid = threadIdx.y * kWX + threadIdx.x;
+    double Uq[NPER][5];
+    bool use[NPER];
+#pragma unroll
+    for (int m = 0; m < NPER; m++) {
+      const int e = tid + m * NT;
+      const int pl = e / (kWLX * kWLY), cc = e % (kWLX * kWLY);
+      const int ly = cc / kWLX, lx = cc % kWLX;
+      use[m] = e < NE && !((lx == 0 || lx == kWLX - 1) && (ly == 0 || ly == kWLY - 1));
+      if (use[m]) {
+        const int gi = wrapm(i0 + lx - 1, g.P1), gj = wrapm(j0 + ly - 1, g.P2);
+        const long long n = (long long)(pl ? k2 : k1) * PL + (long long)gj * g.P1 + gi;
+#pragma unroll
+        for (int c = 0; c < 5; c++) Uq[m][c] = U[c * F + n];
+      }
+    }
+    if (h != 0 && h < step) return;
+#pragma unroll
+    for (int m = 0; m < NPER; m++) {
+      if (!use[m]) continue;
+      const int e = tid + m * NT;
+      const int pl = e / (kWLX * kWLY), cc = e % (kWLX * kWLY);
+      const int ly = cc / kWLX, lx = cc % kWLX;
+      const double r = Uq[m][0];
+      const double ir = 1.0 / r;
+      const double u = Uq[m][1] * ir, v = Uq[m][2] * ir, w = Uq[m][3] * ir;
+      const double T = Uq[m][4] * ir - ph.ggM * 0.5 * (u * u + v * v + w * w);
+      wq[0][pl][ly][lx] = r;
+      wq[1][pl][ly][lx] = u;
+      wq[2][pl][ly][lx] = v;
+      wq[3][pl][ly][lx] = w;
+      wq[4][pl][ly][lx] = T;
+      wq[5][pl][ly][lx] = r * T / ph.gM2;
+      wq[6][pl][ly][lx] = pow07d(T);  // as the stage kernel evaluates mu at these nodes
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  auto prim = [&](int dx, int dy, int pl) {  // column (ii+dx, jj+dy) at plane k1 / k2
+    const int lx = threadIdx.x + 1 + dx, ly = threadIdx.y + 1 + dy;
     P q;
-    q.r = U[n];
-    const double ir = 1.0 / q.r;
-    q.u = U[F + n] * ir;
-    q.v = U[2 * F + n] * ir;
-    q.w = U[3 * F + n] * ir;
-    q.T = U[4 * F + n] * ir - ph.ggM * 0.5 * (q.u * q.u + q.v * q.v + q.w * q.w);
-    q.p = q.r * q.T / ph.gM2;
-    q.mu = pow07d(q.T);  // as the stage kernel evaluates mu at these nodes
+    q.r = wq[0][pl][ly][lx];
+    q.u = wq[1][pl][ly][lx];
+    q.v = wq[2][pl][ly][lx];
+    q.w = wq[3][pl][ly][lx];
+    q.T = wq[4][pl][ly][lx];
+    q.p = wq[5][pl][ly][lx];
+    q.mu = wq[6][pl][ly][lx];
     return q;
   };
   const bool study = (a.wflags >> wall) & 1;
@@ -776,16 +830,13 @@ __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, in
     }
     return o;
   };
-  const int im = wrapi(ii - 1, g.P1), ip = wrapi(ii + 1, g.P1);
-  const int jm = wrapi(jj - 1, g.P2), jp = wrapi(jj + 1, g.P2);
-  const P xp1 = prim(ip, jj, k1), xp2 = prim(ip, jj, k2), xm1 = prim(im, jj, k1), xm2 = prim(im, jj, k2);
-  const P yp1 = prim(ii, jp, k1), yp2 = prim(ii, jp, k2), ym1 = prim(ii, jm, k1), ym2 = prim(ii, jm, k2);
-  const P c1p = prim(ii, jj, k1), c2p = prim(ii, jj, k2);
+  const P xp1 = prim(1, 0, 0), xp2 = prim(1, 0, 1), xm1 = prim(-1, 0, 0), xm2 = prim(-1, 0, 1);
+  const P yp1 = prim(0, 1, 0), yp2 = prim(0, 1, 1), ym1 = prim(0, -1, 0), ym2 = prim(0, -1, 1);
+  const P c1p = prim(0, 0, 0), c2p = prim(0, 0, 1);
   const int ri = ii + 2, rj = jj + 2;  // reference indices of this node
   const WV w0 = wallval(ri, rj, c1p, c2p);
   const WV wxp = wallval(ri + 1, rj, xp1, xp2), wxm = wallval(ri - 1, rj, xm1, xm2);
   const WV wyp = wallval(ri, rj + 1, yp1, yp2), wym = wallval(ri, rj - 1, ym1, ym2);
-  const double pold = pw[wall * PL + n2d];
   const double r = ph.gM2 * pold / w0.T;
   rw[wall * PL + n2d] = r;
   if (check && (r < 0.0 || w0.T < 0.0)) flag(ri, rj);
@@ -848,18 +899,16 @@ __device__ __forceinline__ void wall_node(const Inst& a, int stage, int step, in
 }
 
 __global__ void __launch_bounds__(256) k_wall_fast(Inst a, int stage, int step) {
-  wall_node(a, stage, step, blockIdx.x * blockDim.x + threadIdx.x,
-            blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z);
+  wall_tile(a, stage, step, blockIdx.z);
 }
 
 // all instances of a batch: blockIdx.z = 2 * instance + wall
 __global__ void __launch_bounds__(256) k_wall_batched(const Inst* __restrict__ insts, int stage,
                                                       int step) {
-  wall_node(insts[blockIdx.z >> 1], stage, step, blockIdx.x * blockDim.x + threadIdx.x,
-            blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z & 1);
+  wall_tile(insts[blockIdx.z >> 1], stage, step, blockIdx.z & 1);
 }
 
-// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (161 KB of
+// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (141 KB of
 // LDS, one block per CU); FP32 also has the smaller shapes the batch
 // autotuner chooses between.
 void tile_of(int variant, int& tx, int& ty) {
