@@ -260,10 +260,11 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 
   const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
   const int tlast = min(n3 - 1, tend);             // last plane loaded
-  // The mixed kernel has registers to spare (FP64 runs at 256): it loads
-  // the next plane's state after phase 3 and stage 2's base state at the
-  // top of the iteration, so that a phase of arithmetic hides each global
-  // load's latency.  FP64 loads both where they are used.
+  // The mixed kernel has registers to spare (FP64 runs at 256): right after
+  // the barrier that ends phase 3 it loads the next plane's state and stage
+  // 2's base state for phase 4, so that phases 3b and 4 hide their latency.
+  // (A barrier waits for every load in flight, so a load only overlaps the
+  // work before the next barrier.)  FP64 loads both where they are used.
   constexpr bool kAhead = sizeof(T) == 4;
   double Un1[5], Ur1[5], Ub[5];
   if constexpr (kAhead) {
@@ -290,13 +291,6 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     // plane t-2 of the own column leaves the LDS slot this iteration
 #pragma unroll
     for (int c = 0; c < 4; c++) zU[c] = sl[s][c][r_own];
-    if constexpr (kAhead) {
-      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
-        const long long nb = (long long)(t - 2) * PL + o_own;
-#pragma unroll
-        for (int c = 0; c < 5; c++) Ub[c] = Ubase[c * F + nb];
-      }
-    }
     if (have_plane) {
       T u, v, w, Tt, p, mu;
       double Uo[5];
@@ -571,6 +565,11 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     __syncthreads();
     if constexpr (kAhead) {
       if (t + 1 <= tlast) load_plane(t + 1);
+      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
+        const long long nb = (long long)(t - 2) * PL + o_own;
+#pragma unroll
+        for (int c = 0; c < 5; c++) Ub[c] = Ubase[c * F + nb];
+      }
     }
     if (need_fg) {
       const T* fx = sfg;
