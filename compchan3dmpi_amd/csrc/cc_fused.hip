@@ -263,8 +263,9 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
   // the barrier that ends phase 3 it loads the next plane's state and stage
   // 2's base state for phase 4, so that phases 3b and 4 hide their latency.
-  // (A barrier waits for every load in flight, so a load only overlaps the
-  // work before the next barrier.)  FP64 loads both where they are used.
+  // Loads complete in order: a base-state load issued at the top of phase 1
+  // would be waited for there, with the next-plane values phase 1 consumes.
+  // FP64 loads both where they are used.
   constexpr bool kAhead = sizeof(T) == 4;
   double Un1[5], Ur1[5], Ub[5];
   if constexpr (kAhead) {
