@@ -39,6 +39,16 @@ __device__ __forceinline__ T quickT(T ue, T uw, T qmm, T qm, T qc, T qp, T qpp) 
   return ue * fe - uw * fw;
 }
 
+// Per-plane arrays that no kernel writes while a stage runs (the metric
+// jac/invJac, the forcing tau/rho-bar): read through the constant address
+// space, a uniform index becomes a scalar load.  Its wait then counts scalar
+// loads only, not the vector loads in flight, which complete in order: a
+// vector load of one metric value would have waited for every state load
+// issued before it.
+__device__ __forceinline__ double cread(const double* p, long long i) {
+  return ((const __attribute__((address_space(4))) double*)p)[i];
+}
+
 template <typename T>
 struct Deriv {
   T dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, dTdx, dTdy, dTdz;
@@ -323,7 +333,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 #pragma unroll
       for (int c = 0; c < 5; c++) Uw[4][c] = Uo[c];
       // z scheme of this plane (main_v1.9.f90:409-416)
-      const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(a.jac[t]) > T(ph.Pez) * mu;
+      const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(cread(a.jac, t)) > T(ph.Pez) * mu;
       zq = (zq << 1) | (qz ? 1u : 0u);
       if (has_ring) {
         if constexpr (kAhead) {
@@ -430,7 +440,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     auto zder = [&](T qm, T q0, T qp) -> T {
       if (p == 0) return (qp - q0) / T(ph.dz1);
       if (p == n3 - 1) return (q0 - qm) / T(ph.dzN);
-      return (qp - qm) * (half * invdXi * T(a.invJac[p]));
+      return (qp - qm) * (half * invdXi * T(cread(a.invJac, p)));
     };
     // derivatives at a region node of plane p; z values supplied
     auto derivs = [&](int r, const T zm[3], const T zp[3], T zTm, T zTp, Deriv<T>& D) {
@@ -474,12 +484,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       // iteration's phase 4: Sg[0]) and of plane p+1 (begun in this
       // iteration: Sg[2]); no plane of hD needs to be kept
       if (p - 1 >= kb && p - 1 < ke) {
-        const T cm = half * invdXi * T(a.invJac[p - 1]);
+        const T cm = half * invdXi * T(cread(a.invJac, p - 1));
 #pragma unroll
         for (int c = 0; c < 4; c++) Sg[0][c + 1] += h[c] * cm;
       }
       if (p + 1 >= kb && p + 1 < ke) {
-        const T cp = half * invdXi * T(a.invJac[p + 1]);
+        const T cp = half * invdXi * T(cread(a.invJac, p + 1));
 #pragma unroll
         for (int c = 0; c < 4; c++) Sg[2][c + 1] -= h[c] * cp;
       }
@@ -586,7 +596,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     // ---- phase 4: complete plane q = t-2 ------------------------------
     const int q = t - 2;
     if (q >= kb && q < ke) {
-      const T iJ = T(a.invJac[q]);
+      const T iJ = T(cread(a.invJac, q));
       // z convective (main_v1.9.f90:409-450); column values at q-2..q+2 are
       // window slots 0..4 (slot 2 = q)
       T H[5];
@@ -608,8 +618,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       }
       // (the z divergence of hD is already in Sg[0])
       double R[5];
-      const double tdiff = a.force[0] - a.force[1];
-      const double rb = a.force[2 + q];
+      const double tdiff = cread(a.force, 0) - cread(a.force, 1);
+      const double rb = cread(a.force, 2 + q);
       const double src1 = 0.5 * tdiff * rb;
       // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
       const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
