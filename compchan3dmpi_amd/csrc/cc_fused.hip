@@ -155,6 +155,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   __shared__ T sub[5 * NR];     // U(5) of plane t (phase 2)
   __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
+  // FP64 L2 warming: LDS-DMA loads that nothing reads pull the cache lines of
+  // the next plane and of stage 2's base state into L2 while phases 2 and 3
+  // compute; the barrier after phase 3 waits for them.  One 4-byte word per
+  // node and field lands here.
+  constexpr int NSINK = sizeof(T) == 8 ? NT : 1;
+  __shared__ unsigned int s_sink[NSINK];
 
   const Geom g = a.g;
   const Phys& ph = a.ph;
@@ -379,6 +385,29 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       Sg[2][c] = T(0);
     }
     __syncthreads();
+    if constexpr (!kAhead) {
+      const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+      if (t + 1 <= tlast) {  // plane t+1: own and ring nodes
+        const long long n = (long long)(t + 1) * PL;
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          __builtin_amdgcn_global_load_lds((const unsigned int*)(Uin + c * F + n + o_own),
+                                           &s_sink[wave * 64], 4, 0, 0);
+        if (has_ring) {
+#pragma unroll
+          for (int c = 0; c < 5; c++)
+            __builtin_amdgcn_global_load_lds((const unsigned int*)(Uin + c * F + n + o_ring),
+                                             &s_sink[wave * 64], 4, 0, 0);
+        }
+      }
+      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {  // base state of phase 4
+        const long long nb = (long long)(t - 2) * PL + o_own;
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          __builtin_amdgcn_global_load_lds((const unsigned int*)(Ubase + c * F + nb),
+                                           &s_sink[wave * 64], 4, 0, 0);
+      }
+    }
 
     // ---- phase 2: x, y convective fluxes of plane t (main_v1.9.f90:333-407)
     if (have_plane && t >= kb && t < ke) {
