@@ -24,6 +24,12 @@
 //      RK update.
 // HBM traffic per node and stage: U in (40 B), U out (40 B), plus the base
 // state in stage 2 (40 B) and the x/y halo re-reads, which hit L2.
+//
+// Latency.  Phases 2 and 3 share no barrier (fD/gD have their own LDS).
+// The per-plane metric and forcing values are scalar loads (cread), so no
+// vector-load wait drains the state loads in flight.  FP32 prefetches the
+// next plane and stage 2's base state into registers after phase 3; FP64,
+// out of registers, warms L2 with unread LDS-DMA loads during phases 2-3.
 #include "cc_device.h"
 #include "cc_kernels.h"
 
@@ -947,9 +953,9 @@ __global__ void __launch_bounds__(256) k_wall_batched(const Inst* __restrict__ i
   wall_tile(insts[blockIdx.z >> 1], stage, step, blockIdx.z & 1);
 }
 
-// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (141 KB of
+// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (143 KB of
 // LDS, one block per CU); FP32 also has the smaller shapes the batch
-// autotuner chooses between.
+// autotuner and choose_tiling choose between.
 void tile_of(int variant, int& tx, int& ty) {
   static const int T[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 8}};
   const int v = (variant >= 0 && variant < 4) ? variant : 0;
@@ -958,7 +964,8 @@ void tile_of(int variant, int& tx, int& ty) {
 }
 
 // Fill the tiling fields of an instance for a tile shape and z-chunk
-// (kchunk <= 0: enough blocks to cover the chip at one block per CU).
+// (kchunk <= 0: about 512 blocks of at least 8 planes, the batch autotuner's
+// default candidate; single instances take choose_tiling's choice).
 void set_tiling(Inst& a, int variant, int kchunk) {
   int TX, TY;
   tile_of(variant, TX, TY);
