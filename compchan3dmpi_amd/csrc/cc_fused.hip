@@ -440,26 +440,24 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
           }
         }
       }
-      // y
+      // y: the Peclet test differs from node to node within most waves, so
+      // both schemes are evaluated from one read of the stencil and the
+      // node's own is selected (a divergent branch would read it twice)
       {
         const T rv = U5[2 * NR + ro];
         const T* vl = sl[s][1];
-        if (T(ph.Re) * fabs(rv) * T(ph.dy) > T(ph.Pey) * mu) {
-          const T t1 = (vl[ro] + vl[ro + RX]) * half, t2 = (vl[ro] + vl[ro - RX]) * half;
+        const bool quick = T(ph.Re) * fabs(rv) * T(ph.dy) > T(ph.Pey) * mu;
+        const T vm2 = vl[ro - 2 * RX], vm1 = vl[ro - RX], v0 = vl[ro], vp1 = vl[ro + RX],
+                vp2 = vl[ro + 2 * RX];
+        const T t1 = (v0 + vp1) * half, t2 = (v0 + vm1) * half;
 #pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const T* q = U5 + c * NR + ro;
-            Sg[2][c] += quickT<T>(t1, t2, q[-2 * RX], q[-RX], q[0], q[RX], q[2 * RX]) *
-                        (invdy * T(0.125));
-          }
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const T* q = U5 + c * NR + ro;
-            Sg[2][c] += (-vl[ro + 2 * RX] * q[2 * RX] + T(8) * vl[ro + RX] * q[RX] -
-                         T(8) * vl[ro - RX] * q[-RX] + vl[ro - 2 * RX] * q[-2 * RX]) *
-                        (invdy * T(kOne12th));
-          }
+        for (int c = 0; c < 5; c++) {
+          const T* q = U5 + c * NR + ro;
+          const T qm2 = q[-2 * RX], qm1 = q[-RX], q0 = q[0], qp1 = q[RX], qp2 = q[2 * RX];
+          const T fq = quickT<T>(t1, t2, qm2, qm1, q0, qp1, qp2) * (invdy * T(0.125));
+          const T fc = (-vp2 * qp2 + T(8) * vp1 * qp1 - T(8) * vm1 * qm1 + vm2 * qm2) *
+                       (invdy * T(kOne12th));
+          Sg[2][c] += quick ? fq : fc;
         }
       }
     }
