@@ -161,12 +161,11 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   __shared__ T sub[5 * NR];     // U(5) of plane t (phase 2)
   __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
-  // FP64 L2 warming: LDS-DMA loads that nothing reads pull the cache lines of
-  // the next plane and of stage 2's base state into L2 while phases 2 and 3
-  // compute; the barrier after phase 3 waits for them.  One 4-byte word per
-  // node and field lands here.
-  constexpr int NSINK = sizeof(T) == 8 ? NT : 1;
-  __shared__ unsigned int s_sink[NSINK];
+  // L2 warming: LDS-DMA loads that nothing reads pull the cache lines of the
+  // next plane (FP64, which has no registers to prefetch it) and of stage 2's
+  // base state into L2 while phases 2 and 3 compute; the barrier after phase
+  // 3 waits for them.  One 4-byte word per node and field lands here.
+  __shared__ unsigned int s_sink[NT];
 
   const Geom g = a.g;
   const Phys& ph = a.ph;
@@ -391,9 +390,9 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       Sg[2][c] = T(0);
     }
     __syncthreads();
-    if constexpr (!kAhead) {
+    {
       const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-      if (t + 1 <= tlast) {  // plane t+1: own and ring nodes
+      if (!kAhead && t + 1 <= tlast) {  // plane t+1: own and ring nodes (FP64)
         const long long n = (long long)(t + 1) * PL;
 #pragma unroll
         for (int c = 0; c < 5; c++)
