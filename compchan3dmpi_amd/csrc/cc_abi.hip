@@ -517,6 +517,7 @@ int cc_set_closure(void* handle, void* mlp, const float* norm) {
   c->mlp = mlp;
   if (!mlp) return 0;
   const size_t NI = (size_t)c->g.plane * (c->n3 - 2);
+  if (NI > (size_t)0x7fffffff) return fail(CC_EINVAL, "closure: too many interior nodes (> 2^31)");
   if (!c->cnorm) HIPCHK(hipMalloc((void**)&c->cnorm, 40 * sizeof(float)));
   if (!c->cX) HIPCHK(hipMalloc((void**)&c->cX, NI * 16 * sizeof(float)));
   if (!c->cY) HIPCHK(hipMalloc((void**)&c->cY, NI * 16 * sizeof(float)));

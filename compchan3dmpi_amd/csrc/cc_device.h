@@ -42,6 +42,19 @@ constexpr double kOne12th = 1.0 / 12.0;
 
 __device__ __forceinline__ int wrap_m(int i, int P) { return i < 0 ? i + P : i; }
 __device__ __forceinline__ int wrap_p(int i, int P) { return i >= P ? i - P : i; }
+// Arrays that no kernel writes while the reading kernel runs (the metric
+// jac/invJac, the forcing tau/rho-bar, a normalisation vector): read through
+// the constant address space, a uniform index becomes a scalar load.  Its
+// wait then counts scalar loads only, not the vector loads in flight, which
+// complete in order: a vector load of one such value would wait for every
+// state load issued before it.
+__device__ __forceinline__ double cread(const double* p, long long i) {
+  return ((const __attribute__((address_space(4))) double*)p)[i];
+}
+__device__ __forceinline__ float creadf(const float* p, long long i) {
+  return ((const __attribute__((address_space(4))) float*)p)[i];
+}
+
 __device__ __forceinline__ int wrapi(int i, int P) {
   return i < 0 ? i + P : (i >= P ? i - P : i);
 }

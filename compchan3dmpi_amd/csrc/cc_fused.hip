@@ -45,16 +45,6 @@ __device__ __forceinline__ T quickT(T ue, T uw, T qmm, T qm, T qc, T qp, T qpp) 
   return ue * fe - uw * fw;
 }
 
-// Per-plane arrays that no kernel writes while a stage runs (the metric
-// jac/invJac, the forcing tau/rho-bar): read through the constant address
-// space, a uniform index becomes a scalar load.  Its wait then counts scalar
-// loads only, not the vector loads in flight, which complete in order: a
-// vector load of one metric value would have waited for every state load
-// issued before it.
-__device__ __forceinline__ double cread(const double* p, long long i) {
-  return ((const __attribute__((address_space(4))) double*)p)[i];
-}
-
 template <typename T>
 struct Deriv {
   T dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, dTdx, dTdy, dTdz;

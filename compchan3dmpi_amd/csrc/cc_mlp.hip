@@ -437,10 +437,12 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
                                const double* __restrict__ invJac, const double* __restrict__ uw,
                                const double* __restrict__ vw, const float* __restrict__ norm,
                                float* __restrict__ X) {
-  const long long NI = g.plane * (g.n3 - 2);
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // interior nodes < 2^31 (checked when the closure is attached): 32-bit
+  // index arithmetic
+  const int NI = (int)(g.plane * (g.n3 - 2)), PLn = (int)g.plane;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= NI) return;
-  const int ii = (int)(t % g.P1), jj = (int)((t / g.P1) % g.P2), kk = 1 + (int)(t / g.plane);
+  const int kk = 1 + t / PLn, rem = t - (kk - 1) * PLn, jj = rem / g.P1, ii = rem - jj * g.P1;
   const long long F = g.field;
   auto vel = [&](int i, int j, int k, double* o) {
     if (k == 0 || k == g.n3 - 1) {
@@ -451,10 +453,10 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
       return;
     }
     const long long n = (long long)k * g.plane + (long long)j * g.P1 + i;
-    const double r = U[n];
-    o[0] = U[F + n] / r;
-    o[1] = U[2 * F + n] / r;
-    o[2] = U[3 * F + n] / r;
+    const double ir = 1.0 / U[n];
+    o[0] = U[F + n] * ir;
+    o[1] = U[2 * F + n] * ir;
+    o[2] = U[3 * F + n] * ir;
   };
   double xp[3], xm[3], yp[3], ym[3], zp[3], zm[3];
   vel(wrapi(ii + 1, g.P1), jj, kk, xp);
@@ -471,11 +473,12 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
     f[3 * i + 2] = (float)((zp[i] - zm[i]) * fz);
   }
   const long long n = (long long)kk * g.plane + (long long)jj * g.P1 + ii;
-  const double r = U[n], u = U[F + n] / r, v = U[2 * F + n] / r, w = U[3 * F + n] / r;
+  const double r = U[n], ir = 1.0 / r;
+  const double u = U[F + n] * ir, v = U[2 * F + n] * ir, w = U[3 * F + n] * ir;
   f[9] = (float)r;
-  f[10] = (float)(U[4 * F + n] / r - ph.ggM * 0.5 * (u * u + v * v + w * w));
-  float* x = X + t * 16;
-  for (int c = 0; c < 11; c++) x[c] = (f[c] - norm[c]) / norm[11 + c];
+  f[10] = (float)(U[4 * F + n] * ir - ph.ggM * 0.5 * (u * u + v * v + w * w));
+  float* x = X + (long long)t * 16;
+  for (int c = 0; c < 11; c++) x[c] = (f[c] - creadf(norm, c)) / creadf(norm, 11 + c);
   for (int c = 11; c < 16; c++) x[c] = 0.f;
 }
 
@@ -484,10 +487,10 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
 __global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin,
                             const double* __restrict__ invJac, const float* __restrict__ Y,
                             const float* __restrict__ norm, double fac, double* __restrict__ Uout) {
-  const long long NI = g.plane * (g.n3 - 2);
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int NI = (int)(g.plane * (g.n3 - 2)), PLn = (int)g.plane;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= NI) return;
-  const int ii = (int)(t % g.P1), jj = (int)((t / g.P1) % g.P2), kk = 1 + (int)(t / g.plane);
+  const int kk = 1 + t / PLn, rem = t - (kk - 1) * PLn, jj = rem / g.P1, ii = rem - jj * g.P1;
   const long long F = g.field;
   // flux of node (i,j,k) in direction d: comps 0..2 momentum, 3 energy
   auto flux = [&](int i, int j, int k, int d, double* o) {
@@ -500,11 +503,12 @@ __global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin,
     double tau[9], q[3];
     const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // tau_ab from the 6 stored components
     // norm: feature mean[11], feature std[11], target mean[9], target std[9]
-    for (int a = 0; a < 9; a++) tau[a] = (double)y[map[a]] * norm[31 + map[a]] + norm[22 + map[a]];
-    for (int a = 0; a < 3; a++) q[a] = (double)y[6 + a] * norm[31 + 6 + a] + norm[22 + 6 + a];
+    for (int a = 0; a < 9; a++)
+      tau[a] = (double)y[map[a]] * creadf(norm, 31 + map[a]) + creadf(norm, 22 + map[a]);
+    for (int a = 0; a < 3; a++) q[a] = (double)y[6 + a] * creadf(norm, 37 + a) + creadf(norm, 28 + a);
     const long long n = (long long)k * g.plane + (long long)j * g.P1 + i;
-    const double r = Uin[n];
-    const double u[3] = {Uin[F + n] / r, Uin[2 * F + n] / r, Uin[3 * F + n] / r};
+    const double ir = 1.0 / Uin[n];
+    const double u[3] = {Uin[F + n] * ir, Uin[2 * F + n] * ir, Uin[3 * F + n] * ir};
     for (int a = 0; a < 3; a++) o[a] = tau[3 * a + d];
     o[3] = q[d] + ph.ggM * (u[0] * tau[d] + u[1] * tau[3 + d] + u[2] * tau[6 + d]);
   };
