@@ -125,6 +125,8 @@ void setup_mesh(Chan& c) {
   ph.b2d1c2 = one / t;
   ph.dz1 = dz1;
   ph.dzN = dzN;
+  ph.invdz1 = 1.0 / dz1;
+  ph.invdzN = 1.0 / dzN;
   ph.Re = c.prm.Re;
   ph.invRe = 1.0 / c.prm.Re;
   ph.ggM = gama * (gama - 1) * (c.prm.Mac * c.prm.Mac);

@@ -28,6 +28,7 @@ struct Phys {
   double dx, dy, dXi, invdx, invdy, invdXi;
   double Pex, Pey, Pez;
   double dz1, dzN;
+  double invdz1, invdzN;              // their reciprocals (fused kernels)
   double f1d2c0, f1d2c1, f1d2c2, f2d1c0, f2d1c1, f2d1c2;
   double b1d2c0, b1d2c1, b1d2c2, b2d1c0, b2d1c1, b2d1c2;
   double gkRP;                        // gama*invRe*invPr

@@ -28,8 +28,10 @@
 // Latency.  Phases 2 and 3 share no barrier (fD/gD have their own LDS).
 // The per-plane metric and forcing values are scalar loads (cread), so no
 // vector-load wait drains the state loads in flight.  FP32 prefetches the
-// next plane and stage 2's base state into registers after phase 3; FP64,
-// out of registers, warms L2 with unread LDS-DMA loads during phases 2-3.
+// next plane and the base state U^n into registers after phase 3 and keeps
+// its z window in FP32, which holds it to 168 VGPRs and three waves per
+// SIMD; FP64, out of registers, warms L2 with unread LDS-DMA loads during
+// phases 2-3.
 #include "cc_device.h"
 #include "cc_kernels.h"
 
@@ -217,7 +219,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   };
   const int zr_ring = has_ring ? zring(r_ring % RX, r_ring / RX) : -1;
   // register windows along the own column
-  double Uw[5][5];  // U at planes t-4..t (index 4 = t)
+  T Uw[5][5];       // U at planes t-4..t (index 4 = t)
   T ww[5];          // w at t-4..t
   unsigned zq = 0;  // bit b: plane t-b uses QUICK in z (Pe_z > PezVal)
   T zU[4];          // u,v,w,T at plane t-2 (t-1 and t are in the LDS slots)
@@ -226,7 +228,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   for (int p = 0; p < 5; p++) {
     ww[p] = T(0);
 #pragma unroll
-    for (int c = 0; c < 5; c++) Uw[p][c] = 0.0;
+    for (int c = 0; c < 5; c++) Uw[p][c] = T(0);
   }
 #pragma unroll
   for (int p = 0; p < 3; p++) {
@@ -272,11 +274,13 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
   const int tlast = min(n3 - 1, tend);             // last plane loaded
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
-  // the barrier that ends phase 3 it loads the next plane's state and stage
-  // 2's base state for phase 4, so that phases 3b and 4 hide their latency.
-  // Loads complete in order: a base-state load issued at the top of phase 1
-  // would be waited for there, with the next-plane values phase 1 consumes.
-  // FP64 loads both where they are used.
+  // the barrier that ends phase 3 it loads the next plane's state and the
+  // base state U^n of plane t-2 for phase 4, so that phases 3b and 4 hide
+  // their latency.  Its z window Uw is FP32, so the FP64 base of stage 1
+  // comes from that load too (an L2 hit: phase 1 read the plane two
+  // iterations ago).  Loads complete in order: a base-state load issued at
+  // the top of phase 1 would be waited for there, with the next-plane values
+  // phase 1 consumes.  FP64 loads both where they are used.
   constexpr bool kAhead = sizeof(T) == 4;
   double Un1[5], Ur1[5], Ub[5];
   if constexpr (kAhead) {
@@ -332,7 +336,7 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       }
       ww[4] = w;
 #pragma unroll
-      for (int c = 0; c < 5; c++) Uw[4][c] = Uo[c];
+      for (int c = 0; c < 5; c++) Uw[4][c] = T(Uo[c]);
       // z scheme of this plane (main_v1.9.f90:409-416)
       const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(cread(a.jac, t)) > T(ph.Pez) * mu;
       zq = (zq << 1) | (qz ? 1u : 0u);
@@ -460,8 +464,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     const bool need_fg = p >= kb && p < ke;
     // z derivative of a ring-1 or own node at plane p
     auto zder = [&](T qm, T q0, T qp) -> T {
-      if (p == 0) return (qp - q0) / T(ph.dz1);
-      if (p == n3 - 1) return (q0 - qm) / T(ph.dzN);
+      if (p == 0) return (qp - q0) * T(ph.invdz1);
+      if (p == n3 - 1) return (q0 - qm) * T(ph.invdzN);
       return (qp - qm) * (half * invdXi * T(cread(a.invJac, p)));
     };
     // derivatives at a region node of plane p; z values supplied
@@ -598,7 +602,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     __syncthreads();
     if constexpr (kAhead) {
       if (t + 1 <= tlast) load_plane(t + 1);
-      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
+      // the base state of plane t-2 (U^n, in UA, in both stages)
+      if (t - 2 >= kb && t - 2 < ke && own_valid) {
         const long long nb = (long long)(t - 2) * PL + o_own;
 #pragma unroll
         for (int c = 0; c < 5; c++) Ub[c] = Ubase[c * F + nb];
@@ -625,18 +630,18 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       const bool lowhigh = (q + 1 < 3) || (q + 1 > n3 - 2);
       if (lowhigh) {
 #pragma unroll
-        for (int c = 0; c < 5; c++) H[c] = (ww[3] * T(Uw[3][c]) - ww[1] * T(Uw[1][c])) * (invdXi * iJ * half);
+        for (int c = 0; c < 5; c++) H[c] = (ww[3] * Uw[3][c] - ww[1] * Uw[1][c]) * (invdXi * iJ * half);
       } else if (zq & 4u) {
         const T t1 = (ww[2] + ww[3]) * half, t2 = (ww[2] + ww[1]) * half;
 #pragma unroll
         for (int c = 0; c < 5; c++)
-          H[c] = quickT<T>(t1, t2, T(Uw[0][c]), T(Uw[1][c]), T(Uw[2][c]), T(Uw[3][c]), T(Uw[4][c])) *
+          H[c] = quickT<T>(t1, t2, Uw[0][c], Uw[1][c], Uw[2][c], Uw[3][c], Uw[4][c]) *
                  (invdXi * iJ * T(0.125));
       } else {
 #pragma unroll
         for (int c = 0; c < 5; c++)
-          H[c] = (-ww[4] * T(Uw[4][c]) + T(8) * ww[3] * T(Uw[3][c]) - T(8) * ww[1] * T(Uw[1][c]) +
-                  ww[0] * T(Uw[0][c])) * (invdXi * iJ * T(kOne12th));
+          H[c] = (-ww[4] * Uw[4][c] + T(8) * ww[3] * Uw[3][c] - T(8) * ww[1] * Uw[1][c] +
+                  ww[0] * Uw[0][c]) * (invdXi * iJ * T(kOne12th));
       }
       // (the z divergence of hD is already in Sg[0])
       double R[5];
@@ -650,15 +655,22 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
       R[1] += src1;
       R[4] += src4;
       if (ph.Ri != 0.0) {  // buoyancy -Ri rho along z and its work
-        R[3] += -ph.Ri * Uw[2][0];
-        R[4] += -ph.ggM * ph.Ri * Uw[2][3];
+        R[3] += -ph.Ri * (double)Uw[2][0];
+        R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
       }
       if (own_valid) {
         const long long n = (long long)q * PL + o_own;
         double Un[5];
         if (stage == 1) {
 #pragma unroll
-          for (int c = 0; c < 5; c++) Un[c] = Uw[2][c] + 0.5 * ph.dt * R[c];
+          for (int c = 0; c < 5; c++) {
+            // the mixed kernel's window is FP32; its base state comes in
+            // FP64 from the prefetch
+            if constexpr (kAhead)
+              Un[c] = Ub[c] + 0.5 * ph.dt * R[c];
+            else
+              Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
+          }
         } else {
 #pragma unroll
           for (int c = 0; c < 5; c++) {
@@ -690,8 +702,12 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
 // One instance: blocks cover (tiles x z-chunks).  Blocks are dealt to the 8
 // XCDs round-robin; renumber them so that each XCD walks a contiguous range
 // of tiles and their shared halos stay in that XCD's L2.
+// The FP32 kernels are held to 168 VGPRs, three waves per SIMD: three blocks
+// of a 32x8 tile share a CU (128 KB of LDS).  FP64 runs at 256 VGPRs, one
+// 512-thread block per CU, which its 143 KB of LDS allows anyway.
 template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_stage_fused(Inst a, int stage, int step) {
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+k_stage_fused(Inst a, int stage, int step) {
   const int nblk = a.ntx * a.nty * a.nz;
   int bid = blockIdx.x;
   if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
@@ -706,7 +722,11 @@ __global__ void __launch_bounds__(TX* TY) k_stage_fused(Inst a, int stage, int s
 // (instance, tile_x, tile_y, z-chunk) from a counter until the list is
 // exhausted.  The counter only grows: every block ends with exactly one
 // fetch past the list, so after a launch it stands at base + nwork + grid,
-// which the host uses as the next base (no reset between launches).
+// which the host uses as the next base (no reset between launches).  Each
+// item's descriptor comes from memory, which costs this kernel registers the
+// one-instance kernel keeps in its arguments: held to three waves per SIMD
+// like that one, its FP32 build spills 30 VGPRs and the 16-instance ensemble
+// ran at 1.65 instead of 1.25 ms/step, so it keeps two.
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_stage_batched(const Inst* __restrict__ insts,
                                                          const int4* __restrict__ work, int nwork,
@@ -1005,7 +1025,7 @@ int choose_tiling(Inst& a, int mixed, int* ncu_out, int* bpc_out) {
       ncu < 1)
     ncu = 256;
   const int nint = a.g.n3 - 2;
-  static const int kOrder[4] = {1, 0, 2, 3};  // FP32: 32x8 first (two blocks per CU)
+  static const int kOrder[4] = {1, 0, 2, 3};  // FP32: 32x8 first (three blocks per CU)
   long long best = -1;
   int bv = 0, bk = 1;
   for (int i = 0; i < (mixed ? 4 : 1); i++) {

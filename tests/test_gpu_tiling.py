@@ -11,14 +11,14 @@ GRIDS = [(64, 32, 48), (96, 64, 48), (160, 96, 80), (256, 128, 192), (34, 128, 1
          (7, 9, 6), (37, 21, 11)]
 
 # (grid, precision) -> (tile, planes per block, blocks) on a 256-CU MI355X:
-# FP64 32x16 holds one block per CU, FP32 32x8 two.
+# FP64 32x16 holds one block per CU, FP32 32x8 three.
 EXPECTED_256 = {
     ((64, 32, 48), "fp64"): ((32, 16), 1, 184),      # 4 tiles x 46 chunks, one round
     ((64, 32, 48), "mixed"): ((32, 8), 1, 368),
     ((96, 64, 48), "fp64"): ((32, 16), 3, 192),      # 12 tiles x 16 chunks
     ((160, 96, 80), "fp64"): ((32, 16), 10, 240),    # 30 tiles x 8 chunks
     ((256, 128, 192), "fp64"): ((32, 16), 48, 256),  # 64 tiles x 4 chunks, one round
-    ((256, 128, 192), "mixed"): ((32, 8), 48, 512),
+    ((256, 128, 192), "mixed"): ((32, 8), 32, 768),  # 128 tiles x 6 chunks
     ((34, 128, 192), "fp64"): ((32, 16), 6, 256),    # an 8-way x slab of the above
 }
 
