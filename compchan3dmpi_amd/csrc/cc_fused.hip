@@ -28,10 +28,10 @@
 // Latency.  Phases 2 and 3 share no barrier (fD/gD have their own LDS).
 // The per-plane metric and forcing values are scalar loads (cread), so no
 // vector-load wait drains the state loads in flight.  FP32 prefetches the
-// next plane and the base state U^n into registers after phase 3 and keeps
-// its z window in FP32, which holds it to 168 VGPRs and three waves per
-// SIMD; FP64, out of registers, warms L2 with unread LDS-DMA loads during
-// phases 2-3.
+// next plane into registers after phase 3 and keeps its z window in FP32,
+// which holds it to 168 VGPRs and three waves per SIMD; FP64, out of
+// registers, warms the next plane in L2 with unread LDS-DMA loads during
+// phases 2-3.  Both warm the base state U^n that phase 4 reads.
 #include "cc_device.h"
 #include "cc_kernels.h"
 
@@ -132,8 +132,10 @@ __device__ __forceinline__ float pow07<float>(float x) {
 }
 
 
-template <typename T, int TX, int TY>
-__device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const int step,
+// I is Inst (a kernel argument) or Inst in the constant address space (a
+// batched launch's descriptor): either way its fields are scalar loads.
+template <typename T, int TX, int TY, typename I>
+__device__ __forceinline__ void stage_tile(I& a, const int stage, const int step,
                                            const int tile_x, const int tile_y, const int zc) {
   constexpr int NT = TX * TY;
   constexpr int RX = TX + 4, RY = TY + 4, NR = RX * RY;
@@ -154,13 +156,13 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
   // L2 warming: LDS-DMA loads that nothing reads pull the cache lines of the
-  // next plane (FP64, which has no registers to prefetch it) and of stage 2's
+  // next plane (FP64, which has no registers to prefetch it) and of phase 4's
   // base state into L2 while phases 2 and 3 compute; the barrier after phase
   // 3 waits for them.  One 4-byte word per node and field lands here.
   __shared__ unsigned int s_sink[NT];
 
-  const Geom g = a.g;
-  const Phys& ph = a.ph;
+  const auto& g = a.g;
+  const auto& ph = a.ph;
   {
     // a halt set during this launch holds this step, so every thread of the
     // block takes the same branch here
@@ -274,21 +276,18 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
   const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
   const int tlast = min(n3 - 1, tend);             // last plane loaded
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
-  // the barrier that ends phase 3 it loads the next plane's state and the
-  // base state U^n of plane t-2 for phase 4, so that phases 3b and 4 hide
-  // their latency.  Its z window Uw is FP32, so the FP64 base of stage 1
-  // comes from that load too (an L2 hit: phase 1 read the plane two
-  // iterations ago).  Loads complete in order: a base-state load issued at
-  // the top of phase 1 would be waited for there, with the next-plane values
-  // phase 1 consumes.  FP64 loads both where they are used.
+  // the barrier that ends phase 3 it loads the next plane's state, so that
+  // phases 3b and 4 hide its latency.  FP64 loads it where it is used.
+  // Both load the base state U^n of phase 4 there, from L2 (warmed after the
+  // phase-1 barrier).  The mixed z window Uw is FP32, so its stage 1 takes
+  // the FP64 base from memory too (the plane phase 1 read two iterations
+  // ago).  Holding the base state in registers from phase 3b instead spilled
+  // 4 VGPRs at the 168 of three waves per SIMD and was 1.6 % slower.
   constexpr bool kAhead = sizeof(T) == 4;
-  double Un1[5], Ur1[5], Ub[5];
+  double Un1[5], Ur1[5];
   if constexpr (kAhead) {
 #pragma unroll
-    for (int c = 0; c < 5; c++) {
-      Ub[c] = 0.0;
-      Ur1[c] = 0.0;
-    }
+    for (int c = 0; c < 5; c++) Ur1[c] = 0.0;
   }
   auto load_plane = [&](int t) {
     const long long n = (long long)t * PL;
@@ -399,7 +398,8 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
                                              &s_sink[wave * 64], 4, 0, 0);
         }
       }
-      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {  // base state of phase 4
+      // the base state of phase 4 (mixed: both stages)
+      if ((kAhead || stage == 2) && t - 2 >= kb && t - 2 < ke && own_valid) {
         const long long nb = (long long)(t - 2) * PL + o_own;
 #pragma unroll
         for (int c = 0; c < 5; c++)
@@ -602,12 +602,6 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
     __syncthreads();
     if constexpr (kAhead) {
       if (t + 1 <= tlast) load_plane(t + 1);
-      // the base state of plane t-2 (U^n, in UA, in both stages)
-      if (t - 2 >= kb && t - 2 < ke && own_valid) {
-        const long long nb = (long long)(t - 2) * PL + o_own;
-#pragma unroll
-        for (int c = 0; c < 5; c++) Ub[c] = Ubase[c * F + nb];
-      }
     }
     if (need_fg) {
       const T* fx = sfg;
@@ -664,21 +658,15 @@ __device__ __forceinline__ void stage_tile(const Inst& a, const int stage, const
         if (stage == 1) {
 #pragma unroll
           for (int c = 0; c < 5; c++) {
-            // the mixed kernel's window is FP32; its base state comes in
-            // FP64 from the prefetch
+            // the mixed kernel's window is FP32: its FP64 base from memory
             if constexpr (kAhead)
-              Un[c] = Ub[c] + 0.5 * ph.dt * R[c];
+              Un[c] = Ubase[c * F + n] + 0.5 * ph.dt * R[c];
             else
               Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
           }
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; c++) {
-            if constexpr (kAhead)
-              Un[c] = Ub[c] + ph.dt * R[c];
-            else
-              Un[c] = Ubase[c * F + n] + ph.dt * R[c];
-          }
+          for (int c = 0; c < 5; c++) Un[c] = Ubase[c * F + n] + ph.dt * R[c];
         }
 #pragma unroll
         for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
@@ -722,16 +710,12 @@ k_stage_fused(Inst a, int stage, int step) {
 // (instance, tile_x, tile_y, z-chunk) from a counter until the list is
 // exhausted.  The counter only grows: every block ends with exactly one
 // fetch past the list, so after a launch it stands at base + nwork + grid,
-// which the host uses as the next base (no reset between launches).  Each
-// item's descriptor comes from memory, which costs this kernel registers the
-// one-instance kernel keeps in its arguments: held to three waves per SIMD
-// like that one, its FP32 build spills 30 VGPRs and the 16-instance ensemble
-// ran at 1.65 instead of 1.25 ms/step, so it keeps two.
+// which the host uses as the next base (no reset between launches).  FP32
+// is held to three waves per SIMD, as in the one-instance kernel.
 template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_stage_batched(const Inst* __restrict__ insts,
-                                                         const int4* __restrict__ work, int nwork,
-                                                         unsigned* counter, unsigned base,
-                                                         int stage, int step) {
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+k_stage_batched(const Inst* __restrict__ insts, const int4* __restrict__ work, int nwork,
+                unsigned* counter, unsigned base, int stage, int step) {
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)(atomicAdd(counter, 1u) - base);
@@ -742,7 +726,12 @@ __global__ void __launch_bounds__(TX* TY) k_stage_batched(const Inst* __restrict
     if (w >= nwork) break;
     const int4 it = work[w];
     const int inst = __builtin_amdgcn_readfirstlane(it.x);
-    stage_tile<T, TX, TY>(insts[inst], stage, step, __builtin_amdgcn_readfirstlane(it.y),
+    // the descriptor through the constant address space, so that its fields
+    // are scalar loads as for a kernel argument; read through a generic
+    // pointer, the FP64 build spilled 68 VGPRs (now 15) and ran 2x slower
+    const __attribute__((address_space(4))) Inst& ia =
+        ((const __attribute__((address_space(4))) Inst*)insts)[inst];
+    stage_tile<T, TX, TY>(ia, stage, step, __builtin_amdgcn_readfirstlane(it.y),
                           __builtin_amdgcn_readfirstlane(it.z), __builtin_amdgcn_readfirstlane(it.w));
     __syncthreads();
   }
