@@ -278,8 +278,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage, const int step
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
   // the barrier that ends phase 3 it loads the next plane's state, so that
   // phases 3b and 4 hide its latency.  FP64 loads it where it is used.
-  // Both load the base state U^n of phase 4 there, from L2 (warmed after the
-  // phase-1 barrier).  The mixed z window Uw is FP32, so its stage 1 takes
+  // Both read phase 4's base state U^n in phase 4, from L2 (warmed after
+  // the phase-1 barrier).  The mixed z window Uw is FP32, so its stage 1 takes
   // the FP64 base from memory too (the plane phase 1 read two iterations
   // ago).  Holding the base state in registers from phase 3b instead spilled
   // 4 VGPRs at the 168 of three waves per SIMD and was 1.6 % slower.
