@@ -77,7 +77,9 @@ __device__ __forceinline__ float rcp<float>(float x) {
 // 2^(0.7 log2 x) = 2^n exp(f ln 2) with n = rint(.), |f| <= 1/2 and the
 // exponential by its Taylor series to degree 14.  Error <= 2 ulp over the
 // range a temperature can take, which the 1e-12 FP64 tolerance absorbs.
-// Float: the hardware log2/exp2.
+// Float: the hardware log2/exp2 (v_log_f32, v_exp_f32).  The library's
+// log2f/exp2f wrap the same instructions in rescaling for denormal inputs
+// and results (12 more instructions), which a temperature never reaches.
 __device__ __forceinline__ double pow07d(double x) {
   int e;
   double m = frexp(x, &e);
@@ -128,7 +130,7 @@ __device__ __forceinline__ double pow07<double>(double x) {
 }
 template <>
 __device__ __forceinline__ float pow07<float>(float x) {
-  return exp2f(0.7f * log2f(x));
+  return __builtin_amdgcn_exp2f(0.7f * __builtin_amdgcn_logf(x));
 }
 
 
