@@ -262,14 +262,49 @@ def bench_mlp(n=1 << 21, epochs=5, batch=8192):
                     "kernel times come from a kernel trace"}
 
 
+def bench_coarse(steps=500, nP=(64, 32, 48), dt=4e-4, precision="mixed"):
+    """Cost of one coarse step with and without the closure: ms per step of
+    the seeded laminar start on the coarse grid.  The network is a seeded,
+    untrained one whose outputs are scaled by 1e-9 (target standard
+    deviations), so that the coupling kernels run at full cost while the flow
+    stays the laminar one.  The closure's cost does not depend on its
+    weights."""
+    n1, n2, n3 = nP
+    norm = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT),
+                           np.full(NOUT, 1e-9)]).astype(np.float32)
+    mlp = Mlp(seed=1)
+    out = {"grid": list(nP), "steps": steps, "dt": dt, "precision": precision}
+    for name in ("no_closure", "closure"):
+        ch = Channel(n1, n2, n3, dt=dt, istat=10, precision=precision)
+        ch.upload(formats.laminar_ic(n1, n2, n3))
+        if name == "closure":
+            ch.set_closure(mlp, norm)
+        ch.step(20)
+        t0 = time.perf_counter()
+        rc = ch.step(steps)
+        t = time.perf_counter() - t0
+        ch.close()
+        if rc:
+            mlp.close()
+            raise RuntimeError(f"coarse {name}: blow-up at step {rc}")
+        out[name] = {"ms_per_step": 1e3 * t / steps}
+    mlp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--bench-mlp", type=int, default=0, metavar="N",
                     help="time training on N device-resident samples instead")
+    ap.add_argument("--bench-coarse", type=int, default=0, metavar="STEPS",
+                    help="time STEPS coarse 64x32x48 steps with and without the closure instead")
     a = ap.parse_args()
     if a.bench_mlp:
         print(json.dumps(bench_mlp(a.bench_mlp)), flush=True)
+        return
+    if a.bench_coarse:
+        print(json.dumps(bench_coarse(a.bench_coarse)), flush=True)
         return
     out = experiment(a.quick, log=lambda *x: print(*x, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
