@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from compchan3dmpi_amd import ensemble, formats  # noqa: E402
-from compchan3dmpi_amd.channel import Batch, Channel  # noqa: E402
+from compchan3dmpi_amd.channel import Channel  # noqa: E402
 
 METRIC = "cell-updates/s per RK stage (FP64 vs mixed) + % HBM roofline, 1 MI355X"
 NP = (256, 128, 192)
@@ -69,25 +69,9 @@ def run_gpu(var, precision, steps, warmup, profile, keep=False):
 
 def run_ensemble(steps, warmup):
     """The 16-instance seeded ensemble (mixed precision, persistent batched
-    launches, autotuned tiles): returns cell-updates/s per RK stage summed
-    over the members, ms per step and the tuned configuration."""
-    sp = ensemble.spec(16)
-    b = Batch([s for s, _ in sp])
-    for m, (_, v) in zip(b.members, sp):
-        m.upload(v)
-    tuned = b.tune(trial_steps=3)
-    rc = b.step(warmup)
-    if rc:
-        raise RuntimeError(f"ensemble: blow-up at warm-up step {rc}")
-    t0 = time.perf_counter()
-    rc = b.step(steps)
-    t1 = time.perf_counter()
-    if rc:
-        raise RuntimeError(f"ensemble: blow-up at step {rc}")
-    nodes = sum(s["n1"] * s["n2"] * s["n3"] for s, _ in sp)
-    b.close()
-    return {"value": nodes * 2 * steps / (t1 - t0), "ms_per_step": 1000 * (t1 - t0) / steps,
-            "instances": len(sp), "nodes": nodes, "tuned": tuned}
+    launches, autotuned tiles): cell-updates/s per RK stage summed over the
+    members, ms per step and the tuned configuration."""
+    return ensemble.bench(16, "mixed", steps, warmup)
 
 
 def cpu_baseline(var, nsteps):
