@@ -56,7 +56,11 @@ void orc_set_ext(orc_t *o, double Ri, double Ls);
  *         (v1.9_STWNV/v1.9_s1/codes/main_v1.9_s1.f90:609, ..._s3.f90:609-610);
  *   vel = ORC_WALL_DISCS: three discs of radius x2(n2)/10 centred at
  *         x = 2R, y = 2R, 5R, 8R turning at disc_omega
- *         (v1.9_RD/v1.9_rd1/codes/main_v1.9_rd1.f90:611-640);
+ *         (v1.9_RD/v1.9_rd1/codes/main_v1.9_rd1.f90:611-640).  Parity
+ *         unpinned on a disc's axis: the reference's angle function
+ *         (v1.9_RD/.../comdata.f90:180-193) leaves its result unset when a
+ *         coordinate offset is exactly 0; this takes the axis angle there,
+ *         and no tested grid has a node on an axis;
  *   vel = ORC_WALL_STRIPES: on the stripes u, v satisfy the one-sided
  *         du/dz = dv/dz = grad, no-slip elsewhere; stripes = 0 bands
  *         mod(6j/n2,2)==0 (v1.9_groov/v1.9_g1/codes/main_v1.9_g1.f90:609-618),
