@@ -319,6 +319,7 @@ int cc_create(const cc_params* p, void** handle) {
     a.blow = c->blow;
     a.n1 = c->n1;
     a.n2 = c->n2;
+    a.u32 = p->precision == CC_MIXED;
     c->variant = choose_tiling(a, p->precision == CC_MIXED, &c->tile_ncu, &c->tile_bpc);
     c->kchunk = a.kchunk;
   }
@@ -338,6 +339,9 @@ int cc_upload(void* handle, const double* var, int nstep, double time) {
   HIPCHK(hipMemcpyAsync(c->Vfile, var, 7 * full * sizeof(double), hipMemcpyHostToDevice, c->s));
   launch_upload(c->g, c->ph, c->n1, c->n2, c->Vfile, c->UA, c->UB, c->W, c->wallp(), c->rw,
                 c->dwall, c->consts, c->s);
+#ifdef CC_HAVE_FUSED
+  if (c->inst.u32) launch_predictor_f32(c->inst, c->s);
+#endif
   HIPCHK(hipGetLastError());
   // forcing state: zero until the first statistics step (the reference's
   // static initialisation of tauxzL, tauxzU, stat1Spc)

@@ -7,7 +7,9 @@
 // evaluation order is free (FMA contraction allowed), so FP64 results agree
 // with the reference-order path to rounding, not bit for bit.  With
 // T = float the flux and stencil arithmetic is FP32 while the conserved
-// state, the RK accumulation and every reduction stay FP64.
+// state U^n, the RK accumulation and every reduction stay FP64; the stage-1
+// predictor U*, whose only use is stage 2's FP32 stencil input, is stored
+// as FP32 (a kernel per stage).
 //
 // Work decomposition.  A block owns a TX x TY tile of (ii, jj) columns and a
 // chunk of output planes [kb, ke).  It walks planes t = kb-2 .. ke+1; at each
@@ -23,7 +25,8 @@
 //      (register windows along the column), adds the source and writes the
 //      RK update.
 // HBM traffic per node and stage: U in (40 B), U out (40 B), plus the base
-// state in stage 2 (40 B) and the x/y halo re-reads, which hit L2.
+// state in stage 2 (40 B) and the x/y halo re-reads, which hit L2; mixed
+// moves U* as 20 B (stage 1 out, stage 2 in).
 //
 // Latency.  Phases 2 and 3 share no barrier (fD/gD have their own LDS).
 // The per-plane metric and forcing values are scalar loads (cread), so no
@@ -136,8 +139,9 @@ __device__ __forceinline__ float pow07<float>(float x) {
 
 // I is Inst (a kernel argument) or Inst in the constant address space (a
 // batched launch's descriptor): either way its fields are scalar loads.
-template <typename T, int TX, int TY, typename I>
-__device__ __forceinline__ void stage_tile(I& a, const int stage, const int step,
+// S: the stage (1, 2) as a constant, or 0 for the argument stage_arg (FP64).
+template <typename T, int TX, int TY, int S, typename I>
+__device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int step,
                                            const int tile_x, const int tile_y, const int zc) {
   constexpr int NT = TX * TY;
   constexpr int RX = TX + 4, RY = TY + 4, NR = RX * RY;
@@ -171,9 +175,16 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage, const int step
     const int h = *a.halt;
     if (h != 0 && h < step) return;
   }
+  static_assert(sizeof(T) == 8 || S != 0, "the FP32 kernels take the stage as a constant");
+  const int stage = S ? S : stage_arg;
+  // Mixed precision keeps the stage-1 predictor U* in UB as FP32: stage 1
+  // writes it, stage 2 reads it.  Its only use is the FP32 stencil input of
+  // stage 2, which would round it to FP32 on loading anyway.
+  constexpr bool kIn32 = sizeof(T) == 4 && S == 2, kOut32 = sizeof(T) == 4 && S == 1;
+  using UIn = std::conditional_t<kIn32, float, double>;
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const double* __restrict__ Uin = stage == 1 ? a.UA : a.UB;
+  const UIn* __restrict__ Uin = (const UIn*)(stage == 1 ? a.UA : a.UB);
   const double* Ubase = a.UA;
   double* Uout = stage == 1 ? a.UB : a.UA;
   const int i0 = tile_x * TX, j0 = tile_y * TY;
@@ -280,16 +291,15 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage, const int step
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
   // the barrier that ends phase 3 it loads the next plane's state, so that
   // phases 3b and 4 hide its latency.  FP64 loads it where it is used.
-  // Both read phase 4's base state U^n in phase 4, from L2 (warmed after
-  // the phase-1 barrier).  The mixed z window Uw is FP32, so its stage 1 takes
-  // the FP64 base from memory too (the plane phase 1 read two iterations
-  // ago).  Holding the base state in registers from phase 3b instead spilled
-  // 4 VGPRs at the 168 of three waves per SIMD and was 1.6 % slower.
+  // Stage 2 of both reads its base state U^n in phase 4, from L2 (warmed
+  // after the phase-1 barrier).  Stage 1 takes it from the z window: FP64
+  // exactly; in the mixed kernel, whose window is FP32, rounded to FP32,
+  // which the FP32 predictor it produces absorbs.
   constexpr bool kAhead = sizeof(T) == 4;
-  double Un1[5], Ur1[5];
+  UIn Un1[5], Ur1[5];
   if constexpr (kAhead) {
 #pragma unroll
-    for (int c = 0; c < 5; c++) Ur1[c] = 0.0;
+    for (int c = 0; c < 5; c++) Ur1[c] = UIn(0);
   }
   auto load_plane = [&](int t) {
     const long long n = (long long)t * PL;
@@ -400,8 +410,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage, const int step
                                              &s_sink[wave * 64], 4, 0, 0);
         }
       }
-      // the base state of phase 4 (mixed: both stages)
-      if ((kAhead || stage == 2) && t - 2 >= kb && t - 2 < ke && own_valid) {
+      // the base state of phase 4 (stage 2)
+      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
         const long long nb = (long long)(t - 2) * PL + o_own;
 #pragma unroll
         for (int c = 0; c < 5; c++)
@@ -659,19 +669,18 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage, const int step
         double Un[5];
         if (stage == 1) {
 #pragma unroll
-          for (int c = 0; c < 5; c++) {
-            // the mixed kernel's window is FP32: its FP64 base from memory
-            if constexpr (kAhead)
-              Un[c] = Ubase[c * F + n] + 0.5 * ph.dt * R[c];
-            else
-              Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
-          }
+          for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
         } else {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = Ubase[c * F + n] + ph.dt * R[c];
         }
+        if constexpr (kOut32) {
 #pragma unroll
-        for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
+          for (int c = 0; c < 5; c++) ((float*)Uout)[c * F + n] = (float)Un[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
+        }
         if (stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
           // with rho > 0, T < 0 <=> rho^2 T = rho (rho E) - ggM/2 |rho u|^2 < 0
           const double r2T = Un[0] * Un[4] - ph.ggM * 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]);
@@ -695,7 +704,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage, const int step
 // The FP32 kernels are held to 168 VGPRs, three waves per SIMD: three blocks
 // of a 32x8 tile share a CU (128 KB of LDS).  FP64 runs at 256 VGPRs, one
 // 512-thread block per CU, which its 143 KB of LDS allows anyway.
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, int S>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
 k_stage_fused(Inst a, int stage, int step) {
   const int nblk = a.ntx * a.nty * a.nz;
@@ -705,7 +714,7 @@ k_stage_fused(Inst a, int stage, int step) {
   bid /= a.ntx;
   const int tile_y = bid % a.nty;
   const int zc = bid / a.nty;
-  stage_tile<T, TX, TY>(a, stage, step, tile_x, tile_y, zc);
+  stage_tile<T, TX, TY, S>(a, stage, step, tile_x, tile_y, zc);
 }
 
 // Many instances in one persistent launch: each block pulls work items
@@ -714,7 +723,7 @@ k_stage_fused(Inst a, int stage, int step) {
 // fetch past the list, so after a launch it stands at base + nwork + grid,
 // which the host uses as the next base (no reset between launches).  FP32
 // is held to three waves per SIMD, as in the one-instance kernel.
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, int S>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
 k_stage_batched(const Inst* __restrict__ insts, const int4* __restrict__ work, int nwork,
                 unsigned* counter, unsigned base, int stage, int step) {
@@ -733,7 +742,7 @@ k_stage_batched(const Inst* __restrict__ insts, const int4* __restrict__ work, i
     // pointer, the FP64 build spilled 68 VGPRs (now 15) and ran 2x slower
     const __attribute__((address_space(4))) Inst& ia =
         ((const __attribute__((address_space(4))) Inst*)insts)[inst];
-    stage_tile<T, TX, TY>(ia, stage, step, __builtin_amdgcn_readfirstlane(it.y),
+    stage_tile<T, TX, TY, S>(ia, stage, step, __builtin_amdgcn_readfirstlane(it.y),
                           __builtin_amdgcn_readfirstlane(it.z), __builtin_amdgcn_readfirstlane(it.w));
     __syncthreads();
   }
@@ -765,6 +774,8 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
   const int h = *a.halt;
   const int ii = i0 + threadIdx.x, jj = j0 + threadIdx.y;
   const double* __restrict__ U = stage == 1 ? a.UB : a.UA;
+  // after stage 1 of a mixed instance, UB holds U* as FP32
+  const float* __restrict__ Uf = (stage == 1 && a.u32) ? (const float*)a.UB : nullptr;
   double* __restrict__ pw = a.pw;
   double* __restrict__ rw = a.rw;
   const int check = stage == 2, n1 = a.n1, n2 = a.n2;
@@ -809,8 +820,13 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
       if (use[m]) {
         const int gi = wrapm(i0 + lx - 1, g.P1), gj = wrapm(j0 + ly - 1, g.P2);
         const long long n = (long long)(pl ? k2 : k1) * PL + (long long)gj * g.P1 + gi;
+        if (Uf) {
 #pragma unroll
-        for (int c = 0; c < 5; c++) Uq[m][c] = U[c * F + n];
+          for (int c = 0; c < 5; c++) Uq[m][c] = (double)Uf[c * F + n];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Uq[m][c] = U[c * F + n];
+        }
       }
     }
     if (h != 0 && h < step) return;
@@ -981,13 +997,23 @@ void set_tiling(Inst& a, int variant, int kchunk) {
   a.nz = (nint + kc - 1) / kc;
 }
 
-template <typename T, int TX, int TY>
-static int fused_bpc() {
+template <typename T, int TX, int TY, int S>
+static int fused_bpc1() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_fused<T, TX, TY>, TX * TY, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_fused<T, TX, TY, S>, TX * TY, 0) !=
       hipSuccess)
     n = 1;
   return n > 0 ? n : 1;
+}
+// FP64: one kernel for both stages; FP32: the smaller of its two
+template <typename T, int TX, int TY>
+static int fused_bpc() {
+  if constexpr (sizeof(T) == 8) {
+    return fused_bpc1<T, TX, TY, 0>();
+  } else {
+    const int n1 = fused_bpc1<T, TX, TY, 1>(), n2 = fused_bpc1<T, TX, TY, 2>();
+    return n1 < n2 ? n1 : n2;
+  }
 }
 
 // resident blocks per CU of the one-instance stage kernel
@@ -1045,7 +1071,25 @@ int choose_tiling(Inst& a, int mixed, int* ncu_out, int* bpc_out) {
 template <typename T, int TX, int TY>
 static void launch_one(const Inst& a, int stage, int step, hipStream_t s) {
   const unsigned nblk = (unsigned)(a.ntx * a.nty * a.nz);
-  hipLaunchKernelGGL((k_stage_fused<T, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
+  if constexpr (sizeof(T) == 8)
+    hipLaunchKernelGGL((k_stage_fused<T, TX, TY, 0>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
+  else if (stage == 1)
+    hipLaunchKernelGGL((k_stage_fused<T, TX, TY, 1>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
+  else
+    hipLaunchKernelGGL((k_stage_fused<T, TX, TY, 2>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
+}
+
+// UB <- UA as FP32 (mixed precision), every plane: the wall planes, which
+// no stage writes, then hold the frozen wall state in UB's FP32 layout
+__global__ void __launch_bounds__(256) k_predictor_f32(const double* __restrict__ UA,
+                                                       float* __restrict__ UB, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) UB[i] = (float)UA[i];
+}
+void launch_predictor_f32(const Inst& a, hipStream_t s) {
+  const long long n = 5 * a.g.field;
+  hipLaunchKernelGGL(k_predictor_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.UA,
+                     (float*)a.UB, n);
 }
 
 void launch_wall_fast(const Inst& a, int stage, int step, hipStream_t s) {
@@ -1071,13 +1115,22 @@ void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int st
   if (wall) launch_wall_fast(a, stage, step, s);
 }
 
-template <typename T, int TX, int TY>
-static int blocks_per_cu() {
+template <typename T, int TX, int TY, int S>
+static int blocks_per_cu1() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_batched<T, TX, TY>, TX * TY, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_batched<T, TX, TY, S>, TX * TY, 0) !=
       hipSuccess)
     n = 1;
   return n > 0 ? n : 1;
+}
+template <typename T, int TX, int TY>
+static int blocks_per_cu() {
+  if constexpr (sizeof(T) == 8) {
+    return blocks_per_cu1<T, TX, TY, 0>();
+  } else {
+    const int n1 = blocks_per_cu1<T, TX, TY, 1>(), n2 = blocks_per_cu1<T, TX, TY, 2>();
+    return n1 < n2 ? n1 : n2;
+  }
 }
 
 int batched_blocks_per_cu(int mixed, int variant) {
@@ -1090,14 +1143,27 @@ int batched_blocks_per_cu(int mixed, int variant) {
   }
 }
 
+template <typename T, int TX, int TY>
+static void launch_b(const Inst* d_insts, const int4* d_work, int nwork, unsigned* counter,
+                     unsigned base, int nblocks, int stage, int step, hipStream_t s) {
+  if constexpr (sizeof(T) == 8)
+    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 0>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
+                       d_work, nwork, counter, base, stage, step);
+  else if (stage == 1)
+    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 1>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
+                       d_work, nwork, counter, base, stage, step);
+  else
+    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 2>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
+                       d_work, nwork, counter, base, stage, step);
+}
+
 void launch_stage_batched(const Inst* d_insts, const int4* d_work, int nwork, unsigned* counter,
                           unsigned base, int nblocks, int mixed, int variant, int stage, int step,
                           int ninst, int maxP1, int maxP2, hipStream_t s, hipEvent_t ev0,
                           hipEvent_t ev1) {
   if (ev0) (void)hipEventRecord(ev0, s);
-#define CC_LAUNCH_B(T, TX, TY)                                                                  \
-  hipLaunchKernelGGL((k_stage_batched<T, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts, \
-                     d_work, nwork, counter, base, stage, step)
+#define CC_LAUNCH_B(T, TX, TY) \
+  launch_b<T, TX, TY>(d_insts, d_work, nwork, counter, base, nblocks, stage, step, s)
   if (!mixed) {
     CC_LAUNCH_B(double, 32, 16);
   } else {

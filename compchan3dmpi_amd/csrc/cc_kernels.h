@@ -38,7 +38,10 @@ void launch_stage_strict(const Geom& g, const Phys& ph, const double* Uin, const
 
 // ---- cc_fused.hip: fused streaming stage (FP64 or mixed) --------------------
 // One channel instance as the stage kernels see it.  Stage 1 reads UA and
-// writes UB; stage 2 reads UB, takes UA as the RK base and writes UA.
+// writes UB; stage 2 reads UB, takes UA as the RK base and writes UA.  With
+// u32 (mixed precision) UB holds the stage-1 predictor U* as FP32, the
+// precision its only uses (FP32 stencils) take; U^n and U^{n+1} in UA stay
+// FP64.
 struct Inst {
   Geom g;
   Phys ph;
@@ -65,6 +68,7 @@ struct Inst {
   unsigned long long* blow;
   int n1, n2;
   int kchunk, ntx, nty, nz;  // tiling of the stage kernel
+  int u32;                   // UB holds U* as FP32 (mixed precision)
 };
 // tile variants: 0 = 32x16 (the only FP64 one), 1 = 32x8, 2 = 16x16, 3 = 64x8
 void tile_of(int variant, int& tx, int& ty);
@@ -78,6 +82,9 @@ void launch_stage_fused(const Inst& a, int mixed, int variant, int stage, int st
                         bool wall = true);
 // wall values and wall pressure after a stage (part of launch_stage_fused)
 void launch_wall_fast(const Inst& a, int stage, int step, hipStream_t s);
+// mixed precision: UB <- UA as FP32, so that UB's wall planes, which no
+// stage writes, hold the frozen wall state in UB's FP32 layout
+void launch_predictor_f32(const Inst& a, hipStream_t s);
 
 // ---- cc_mlp.hip: learned subgrid closure ------------------------------------
 // Subtract the closure's subgrid-flux divergence from a stage's output; X, Y

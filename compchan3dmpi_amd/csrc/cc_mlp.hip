@@ -433,7 +433,8 @@ __global__ void k_box_samples(int bx, int by, int bz, const double* __restrict__
 
 // ---- coupling to a coarse run -----------------------------------------------
 // Features of every interior node of the coarse state, normalised.
-__global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
+// in32: U is the FP32 predictor U* (stage 2 of a mixed instance).
+__global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U, int in32,
                                const double* __restrict__ invJac, const double* __restrict__ uw,
                                const double* __restrict__ vw, const float* __restrict__ norm,
                                float* __restrict__ X) {
@@ -444,6 +445,7 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
   if (t >= NI) return;
   const int kk = 1 + t / PLn, rem = t - (kk - 1) * PLn, jj = rem / g.P1, ii = rem - jj * g.P1;
   const long long F = g.field;
+  auto ld = [&](long long i) -> double { return in32 ? (double)((const float*)U)[i] : U[i]; };
   auto vel = [&](int i, int j, int k, double* o) {
     if (k == 0 || k == g.n3 - 1) {
       const long long wo = (k == 0 ? 0 : g.plane) + (long long)j * g.P1 + i;
@@ -453,10 +455,10 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
       return;
     }
     const long long n = (long long)k * g.plane + (long long)j * g.P1 + i;
-    const double ir = 1.0 / U[n];
-    o[0] = U[F + n] * ir;
-    o[1] = U[2 * F + n] * ir;
-    o[2] = U[3 * F + n] * ir;
+    const double ir = 1.0 / ld(n);
+    o[0] = ld(F + n) * ir;
+    o[1] = ld(2 * F + n) * ir;
+    o[2] = ld(3 * F + n) * ir;
   };
   double xp[3], xm[3], yp[3], ym[3], zp[3], zm[3];
   vel(wrapi(ii + 1, g.P1), jj, kk, xp);
@@ -473,25 +475,28 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
     f[3 * i + 2] = (float)((zp[i] - zm[i]) * fz);
   }
   const long long n = (long long)kk * g.plane + (long long)jj * g.P1 + ii;
-  const double r = U[n], ir = 1.0 / r;
-  const double u = U[F + n] * ir, v = U[2 * F + n] * ir, w = U[3 * F + n] * ir;
+  const double r = ld(n), ir = 1.0 / r;
+  const double u = ld(F + n) * ir, v = ld(2 * F + n) * ir, w = ld(3 * F + n) * ir;
   f[9] = (float)r;
-  f[10] = (float)(U[4 * F + n] * ir - ph.ggM * 0.5 * (u * u + v * v + w * w));
+  f[10] = (float)(ld(4 * F + n) * ir - ph.ggM * 0.5 * (u * u + v * v + w * w));
   float* x = X + (long long)t * 16;
   for (int c = 0; c < 11; c++) x[c] = (f[c] - creadf(norm, c)) / creadf(norm, 11 + c);
   for (int c = 11; c < 16; c++) x[c] = 0.f;
 }
 
 // Subtract fac*dt * div(subgrid flux) from the stage output: momentum flux
-// tau_ij, energy flux q_j + ggM u_i tau_ij (zero on the walls).
-__global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin,
+// tau_ij, energy flux q_j + ggM u_i tau_ij (zero on the walls).  in32 / out32:
+// the stage input / output is the FP32 predictor U* (mixed instances).
+__global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin, int in32,
                             const double* __restrict__ invJac, const float* __restrict__ Y,
-                            const float* __restrict__ norm, double fac, double* __restrict__ Uout) {
+                            const float* __restrict__ norm, double fac, double* __restrict__ Uout,
+                            int out32) {
   const int NI = (int)(g.plane * (g.n3 - 2)), PLn = (int)g.plane;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= NI) return;
   const int kk = 1 + t / PLn, rem = t - (kk - 1) * PLn, jj = rem / g.P1, ii = rem - jj * g.P1;
   const long long F = g.field;
+  auto ld = [&](long long i) -> double { return in32 ? (double)((const float*)Uin)[i] : Uin[i]; };
   // flux of node (i,j,k) in direction d: comps 0..2 momentum, 3 energy
   auto flux = [&](int i, int j, int k, int d, double* o) {
     if (k == 0 || k == g.n3 - 1) {
@@ -507,8 +512,8 @@ __global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin,
       tau[a] = (double)y[map[a]] * creadf(norm, 31 + map[a]) + creadf(norm, 22 + map[a]);
     for (int a = 0; a < 3; a++) q[a] = (double)y[6 + a] * creadf(norm, 37 + a) + creadf(norm, 28 + a);
     const long long n = (long long)k * g.plane + (long long)j * g.P1 + i;
-    const double ir = 1.0 / Uin[n];
-    const double u[3] = {Uin[F + n] * ir, Uin[2 * F + n] * ir, Uin[3 * F + n] * ir};
+    const double ir = 1.0 / ld(n);
+    const double u[3] = {ld(F + n) * ir, ld(2 * F + n) * ir, ld(3 * F + n) * ir};
     for (int a = 0; a < 3; a++) o[a] = tau[3 * a + d];
     o[3] = q[d] + ph.ggM * (u[0] * tau[d] + u[1] * tau[3 + d] + u[2] * tau[6 + d]);
   };
@@ -523,7 +528,13 @@ __global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin,
   flux(ii, jj, kk - 1, 2, fm);
   for (int c = 0; c < 4; c++) div[c] += (fp[c] - fm[c]) * 0.5 * ph.invdXi * invJac[kk];
   const long long n = (long long)kk * g.plane + (long long)jj * g.P1 + ii;
-  for (int c = 0; c < 4; c++) Uout[(c + 1) * F + n] -= fac * ph.dt * div[c];
+  if (out32) {
+    float* Uf = (float*)Uout;
+    for (int c = 0; c < 4; c++)
+      Uf[(c + 1) * F + n] = (float)((double)Uf[(c + 1) * F + n] - fac * ph.dt * div[c]);
+  } else {
+    for (int c = 0; c < 4; c++) Uout[(c + 1) * F + n] -= fac * ph.dt * div[c];
+  }
 }
 
 }  // namespace cc
@@ -635,13 +646,15 @@ void closure_stage(void* mlp, const Inst& a, const float* norm, float* X, float*
   const long long NI = a.g.plane * (a.g.n3 - 2);
   const double* Uin = stage == 1 ? a.UA : a.UB;
   double* Uout = stage == 1 ? a.UB : a.UA;
+  // mixed instances keep the stage-1 predictor U* (in UB) as FP32
+  const int in32 = a.u32 && stage == 2, out32 = a.u32 && stage == 1;
   const int B = 256;
   hipLaunchKernelGGL(k_les_features, dim3((unsigned)((NI + B - 1) / B)), dim3(B), 0, s, a.g, a.ph,
-                     Uin, a.invJac, a.uw, a.vw, norm, X);
+                     Uin, in32, a.invJac, a.uw, a.vw, norm, X);
   hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((NI + 63) / 64)), dim3(256), 0, s, X,
                      (const int*)nullptr, (int)NI, m->wpk, m->P, Y, (float*)nullptr);
   hipLaunchKernelGGL(k_les_apply, dim3((unsigned)((NI + B - 1) / B)), dim3(B), 0, s, a.g, a.ph,
-                     Uin, a.invJac, Y, norm, stage == 1 ? 0.5 : 1.0, Uout);
+                     Uin, in32, a.invJac, Y, norm, stage == 1 ? 0.5 : 1.0, Uout, out32);
 }
 void box_filter(const Inst& a, int bx, int by, int bz, double* S, const double* zc, float* X,
                 float* T, hipStream_t s) {
