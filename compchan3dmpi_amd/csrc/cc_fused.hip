@@ -35,6 +35,8 @@
 // which holds it to 168 VGPRs and three waves per SIMD; FP64, out of
 // registers, warms the next plane in L2 with unread LDS-DMA loads during
 // phases 2-3.  Both warm the base state U^n that phase 4 reads.
+#include <type_traits>
+
 #include "cc_device.h"
 #include "cc_kernels.h"
 

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cc_abi.h"
@@ -433,8 +434,9 @@ __global__ void k_box_samples(int bx, int by, int bz, const double* __restrict__
 
 // ---- coupling to a coarse run -----------------------------------------------
 // Features of every interior node of the coarse state, normalised.
-// in32: U is the FP32 predictor U* (stage 2 of a mixed instance).
-__global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U, int in32,
+// IN32: U is the FP32 predictor U* (stage 2 of a mixed instance).
+template <bool IN32>
+__global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U,
                                const double* __restrict__ invJac, const double* __restrict__ uw,
                                const double* __restrict__ vw, const float* __restrict__ norm,
                                float* __restrict__ X) {
@@ -445,7 +447,9 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U, in
   if (t >= NI) return;
   const int kk = 1 + t / PLn, rem = t - (kk - 1) * PLn, jj = rem / g.P1, ii = rem - jj * g.P1;
   const long long F = g.field;
-  auto ld = [&](long long i) -> double { return in32 ? (double)((const float*)U)[i] : U[i]; };
+  using UT = std::conditional_t<IN32, float, double>;
+  const UT* __restrict__ Ut = (const UT*)U;
+  auto ld = [&](long long i) -> double { return (double)Ut[i]; };
   auto vel = [&](int i, int j, int k, double* o) {
     if (k == 0 || k == g.n3 - 1) {
       const long long wo = (k == 0 ? 0 : g.plane) + (long long)j * g.P1 + i;
@@ -485,18 +489,20 @@ __global__ void k_les_features(Geom g, Phys ph, const double* __restrict__ U, in
 }
 
 // Subtract fac*dt * div(subgrid flux) from the stage output: momentum flux
-// tau_ij, energy flux q_j + ggM u_i tau_ij (zero on the walls).  in32 / out32:
+// tau_ij, energy flux q_j + ggM u_i tau_ij (zero on the walls).  IN32 / OUT32:
 // the stage input / output is the FP32 predictor U* (mixed instances).
-__global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin, int in32,
+template <bool IN32, bool OUT32>
+__global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin,
                             const double* __restrict__ invJac, const float* __restrict__ Y,
-                            const float* __restrict__ norm, double fac, double* __restrict__ Uout,
-                            int out32) {
+                            const float* __restrict__ norm, double fac, double* __restrict__ Uout) {
   const int NI = (int)(g.plane * (g.n3 - 2)), PLn = (int)g.plane;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= NI) return;
   const int kk = 1 + t / PLn, rem = t - (kk - 1) * PLn, jj = rem / g.P1, ii = rem - jj * g.P1;
   const long long F = g.field;
-  auto ld = [&](long long i) -> double { return in32 ? (double)((const float*)Uin)[i] : Uin[i]; };
+  using UT = std::conditional_t<IN32, float, double>;
+  const UT* __restrict__ Ut = (const UT*)Uin;
+  auto ld = [&](long long i) -> double { return (double)Ut[i]; };
   // flux of node (i,j,k) in direction d: comps 0..2 momentum, 3 energy
   auto flux = [&](int i, int j, int k, int d, double* o) {
     if (k == 0 || k == g.n3 - 1) {
@@ -528,7 +534,7 @@ __global__ void k_les_apply(Geom g, Phys ph, const double* __restrict__ Uin, int
   flux(ii, jj, kk - 1, 2, fm);
   for (int c = 0; c < 4; c++) div[c] += (fp[c] - fm[c]) * 0.5 * ph.invdXi * invJac[kk];
   const long long n = (long long)kk * g.plane + (long long)jj * g.P1 + ii;
-  if (out32) {
+  if constexpr (OUT32) {
     float* Uf = (float*)Uout;
     for (int c = 0; c < 4; c++)
       Uf[(c + 1) * F + n] = (float)((double)Uf[(c + 1) * F + n] - fac * ph.dt * div[c]);
@@ -649,12 +655,25 @@ void closure_stage(void* mlp, const Inst& a, const float* norm, float* X, float*
   // mixed instances keep the stage-1 predictor U* (in UB) as FP32
   const int in32 = a.u32 && stage == 2, out32 = a.u32 && stage == 1;
   const int B = 256;
-  hipLaunchKernelGGL(k_les_features, dim3((unsigned)((NI + B - 1) / B)), dim3(B), 0, s, a.g, a.ph,
-                     Uin, in32, a.invJac, a.uw, a.vw, norm, X);
+  const dim3 grid((unsigned)((NI + B - 1) / B));
+  if (in32)
+    hipLaunchKernelGGL(k_les_features<true>, grid, dim3(B), 0, s, a.g, a.ph, Uin, a.invJac, a.uw,
+                       a.vw, norm, X);
+  else
+    hipLaunchKernelGGL(k_les_features<false>, grid, dim3(B), 0, s, a.g, a.ph, Uin, a.invJac, a.uw,
+                       a.vw, norm, X);
   hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((NI + 63) / 64)), dim3(256), 0, s, X,
                      (const int*)nullptr, (int)NI, m->wpk, m->P, Y, (float*)nullptr);
-  hipLaunchKernelGGL(k_les_apply, dim3((unsigned)((NI + B - 1) / B)), dim3(B), 0, s, a.g, a.ph,
-                     Uin, in32, a.invJac, Y, norm, stage == 1 ? 0.5 : 1.0, Uout, out32);
+  const double fac = stage == 1 ? 0.5 : 1.0;
+  if (in32)
+    hipLaunchKernelGGL((k_les_apply<true, false>), grid, dim3(B), 0, s, a.g, a.ph, Uin, a.invJac, Y,
+                       norm, fac, Uout);
+  else if (out32)
+    hipLaunchKernelGGL((k_les_apply<false, true>), grid, dim3(B), 0, s, a.g, a.ph, Uin, a.invJac, Y,
+                       norm, fac, Uout);
+  else
+    hipLaunchKernelGGL((k_les_apply<false, false>), grid, dim3(B), 0, s, a.g, a.ph, Uin, a.invJac, Y,
+                       norm, fac, Uout);
 }
 void box_filter(const Inst& a, int bx, int by, int bz, double* S, const double* zc, float* X,
                 float* T, hipStream_t s) {
