@@ -138,7 +138,6 @@ __device__ __forceinline__ float pow07<float>(float x) {
   return __builtin_amdgcn_exp2f(0.7f * __builtin_amdgcn_logf(x));
 }
 
-
 // I is Inst (a kernel argument) or Inst in the constant address space (a
 // batched launch's descriptor): either way its fields are scalar loads.
 // S: the stage (1, 2) as a constant, or 0 for the argument stage_arg (FP64).
@@ -150,7 +149,6 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   constexpr int NRING = NR - NT;                // halo-2 ring points
   constexpr int NFX = (TX + 2) * TY;            // fD points (tile + x halo 1)
   constexpr int NGY = TX * (TY + 2);            // gD points (tile + y halo 1)
-  constexpr int NFG = NFX > NGY ? NFX : NGY;
   constexpr int NZR = 2 * TY + 2 * TX;          // x- and y- ring-1 points
   static_assert(NRING <= NT, "one ring node per thread");
   // first threads of the y-ring (gD) and x-ring (fD) point-flux passes
