@@ -2,7 +2,6 @@
 reference build) on the same seeded inputs.  Tolerances: the north-star bar is
 relative L2 <= 1e-12 in the conserved variables after N steps in FP64."""
 import os
-import sys
 
 import numpy as np
 import pytest

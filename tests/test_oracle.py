@@ -4,7 +4,6 @@ import hashlib
 import os
 
 import numpy as np
-import pytest
 
 from compchan3dmpi_amd import formats
 import oracle as orc
