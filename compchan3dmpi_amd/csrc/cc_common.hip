@@ -79,7 +79,7 @@ __device__ __forceinline__ int ring_slot(int n1, int n2, int P1, int P2, int i, 
 }
 
 // ---- upload: conserved state from var (main_v1.9.f90:284-294) -----------
-__global__ void k_upload(Geom g, Phys ph, int n1, int n2, const double* __restrict__ V,
+__global__ void k_upload(Geom g, Phys, int n1, int n2, const double* __restrict__ V,
                          double* __restrict__ UA, double* __restrict__ UB, double* __restrict__ W,
                          WallP pw, double* __restrict__ rw) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
