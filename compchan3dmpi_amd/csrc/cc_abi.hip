@@ -249,6 +249,8 @@ int cc_create(const cc_params* p, void** handle) {
   *handle = nullptr;
   if (p->n1 < 6 || p->n2 < 6 || p->n3 < 6)
     return fail(CC_EINVAL, "grid too small: every nP must be >= 6");
+  if ((long long)(p->n1 - 2) * (p->n2 - 2) > 0x7fffffffLL)
+    return fail(CC_EINVAL, "grid too large: (n1-2)(n2-2) must be below 2^31");
   if (p->istat < 1) return fail(CC_EINVAL, "istat must be >= 1");
   if (!(p->dt > 0.0) || !(p->Re > 0.0) || !(p->Mac > 0.0))
     return fail(CC_EINVAL, "dt, Re and Mac must be positive");

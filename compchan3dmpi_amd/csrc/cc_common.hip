@@ -343,9 +343,11 @@ __global__ void __launch_bounds__(256) k_planestats_part(Geom g, Phys ph,
   double acc[kNStat];
   for (int q = 0; q < kNStat; q++) acc[q] = 0.0;
   double tq[kNStat];
-  const long long n0 = g.plane * sp / kStatSplit, n1e = g.plane * (sp + 1) / kStatSplit;
-  for (long long n = n0 + threadIdx.x; n < n1e; n += blockDim.x) {
-    const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
+  // a plane holds fewer than 2^31 nodes (checked at create): 32-bit index
+  // arithmetic
+  const int n0 = (int)(g.plane * sp / kStatSplit), n1e = (int)(g.plane * (sp + 1) / kStatSplit);
+  for (int n = n0 + (int)threadIdx.x; n < n1e; n += blockDim.x) {
+    const int jj = n / g.P1, ii = n - jj * g.P1;
     Prim P;
     double wgt = 1.0;
     if (wall) {
