@@ -42,6 +42,16 @@ def test_create_rejects_bad_arguments():
     assert b"grid too small" in L.cc_last_error()
     p = channel.cc_params(12, 8, 16, 10, 7, 0, 3000.0, 1.5, 1e-4, 2.0, 2.0, 0.2, 0.0, 0.0)
     assert L.cc_create(C.byref(p), C.byref(h)) < 0
+    # a plane of 2^31 or more unique nodes: the kernels index within a plane
+    # in 32-bit arithmetic
+    p = channel.cc_params(50002, 50002, 16, 10, 1, 0, 3000.0, 1.5, 1e-4, 2.0, 2.0, 0.2, 0.0, 0.0)
+    assert L.cc_create(C.byref(p), C.byref(h)) < 0
+    assert b"grid too large" in L.cc_last_error()
+    # (n1-2)(n2-2) = 46340^2 < 2^31 passes the size check: with dt = 0 the
+    # next check rejects it, before any device call
+    p = channel.cc_params(46342, 46342, 16, 10, 1, 0, 3000.0, 1.5, 0.0, 2.0, 2.0, 0.2, 0.0, 0.0)
+    assert L.cc_create(C.byref(p), C.byref(h)) < 0
+    assert b"must be positive" in L.cc_last_error()
 
 
 def test_no_device_is_an_error_not_a_fallback():
