@@ -9,13 +9,18 @@ step: two RK stages plus the istat-cadence wall-shear and plane statistics
 that feed the forcing, all on the device.
 
 Prints one JSON line.  value = cell-updates/s per RK stage in FP64
-(= n1*n2*n3 * 2 * steps / t); the mixed-precision figure is reported in
-config.mixed.  roofline: achieved algorithmic HBM bandwidth of the stage
-kernel (100 B per node per stage: stage 1 reads U^n and writes U*, stage 2
-reads U*, U^n and writes U^{n+1}; 5 FP64 words each) over its average launch
-time measured with HIP events on the library's stream.  cpu_baseline: the
-CPU restatement (oracle/, its node loops on every OpenMP thread) on a bounded
-sample of the same grid.
+(= n1*n2*n3 * 2 * steps / t: the metric's node count of 6.3M for this grid,
+periodic duplicates and wall planes included; config.cells also gives the
+(n1-2)(n2-2)(n3-2) nodes a stage actually updates); the mixed-precision
+figure is reported in config.mixed, with its own roofline and the per-field
+error against FP64 after 1000 steps (config 3), whatever --steps is.
+roofline: achieved algorithmic HBM bandwidth of the stage kernel over its
+average launch time measured with HIP events on the library's stream.  FP64
+moves 100 B per node and stage (stage 1 reads U^n and writes U*, stage 2
+reads U*, U^n and writes U^{n+1}; 5 FP64 words each); mixed keeps U* in
+FP32, 80 B.  cpu_baseline: the reference itself (oracle/_ref) on one MPI rank
+per host core the process may use, or the CPU restatement (oracle/) where
+that build is absent, on a bounded sample of the same grid.
 """
 from __future__ import annotations
 
@@ -38,7 +43,9 @@ NP = (256, 128, 192)
 DT = 9e-5           # CFL 0.5 on the seeded start, five decimals (formats.cfl_dt)
 ISTAT = 10
 HBM_PEAK_GBS = 8000.0
-ALG_BYTES_PER_NODE_STAGE = 100.0
+ALG_BYTES_PER_NODE_STAGE = 100.0        # FP64: 40 B in, 40 B out, + 40 B base state in stage 2
+ALG_BYTES_PER_NODE_STAGE_MIXED = 80.0   # U* as FP32: stage 1 40 + 20, stage 2 20 + 40 + 40
+ERR_STEPS = 1000                        # config 3: per-field error after 1000 steps
 
 
 def log(*a):
@@ -91,9 +98,46 @@ def cpu_baseline(var, nsteps):
 REF_STEPS = 20  # compile-time maxstep of the reference build timed as the CPU baseline
 
 
+def host_cores():
+    """Host cores this process may use: the CPU affinity set, capped by a
+    cgroup CPU quota where one is set (a container's CPU share), with
+    os.cpu_count() reported beside it."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    usable = min(aff, quota) if quota else aff
+    return {"usable": usable, "nproc": os.cpu_count(), "affinity": aff, "cgroup_quota": quota}
+
+
+def topology(nr, n):
+    """A 3-D process grid of nr ranks for nP = n: the factorisation whose
+    sub-blocks are closest to cubes (smallest halo surface per rank)."""
+    best = None
+    for a in range(1, nr + 1):
+        if nr % a:
+            continue
+        for b in range(1, nr // a + 1):
+            if (nr // a) % b:
+                continue
+            c = nr // a // b
+            if n[0] // a < 4 or n[1] // b < 4 or n[2] // c < 4:
+                continue
+            bx, by, bz = n[0] / a, n[1] / b, n[2] / c
+            surf = bx * by + by * bz + bx * bz
+            if best is None or surf < best[0]:
+                best = (surf, (a, b, c))
+    return best[1] if best else (1, 1, 1)
+
+
 def cpu_reference(var):
     """The reference itself (oracle/_ref, built from its sources by
-    oracle/build_ref.sh, one MPI rank per host core up to 16) on the same
+    oracle/build_ref.sh, one MPI rank per usable host core) on the same
     grid and start for REF_STEPS steps.  Its rate comes from its own
     10-step CPU-time line (main_v1.9.f90:1066-1067) of the last, warm block.
     Returns None where that build or MPI is not present."""
@@ -108,10 +152,9 @@ def cpu_reference(var):
     mpiexec = "/opt/conda/bin/mpiexec"
     if not (os.path.exists(exe) and os.path.exists(mpiexec)):
         return None
-    ncpu = min(16, len(os.sched_getaffinity(0)))
-    topo = {16: (4, 2, 2), 8: (2, 2, 2), 4: (2, 2, 1), 2: (2, 1, 1)}
-    nr = max(k for k in topo if k <= ncpu) if ncpu >= 2 else 1
-    tp = topo.get(nr, (1, 1, 1))
+    cores = host_cores()
+    nr = min(cores["usable"], 128)  # each rank holds the whole var array (resOp 1)
+    tp = topology(nr, NP)
     root = tempfile.mkdtemp(prefix="ccref_")
     try:
         x1, x2, x3 = formats.mesh(n1, n2, n3)[:3]
@@ -129,7 +172,7 @@ def cpu_reference(var):
             return None
         sec = float(blocks[-1])
         return {"value": n1 * n2 * n3 * 2 * 10 / sec, "unit": "cell-updates/s", "cores": nr,
-                "kind": "reference",
+                "kind": "reference", "host": cores,
                 "sample": f"{REF_STEPS} steps of nP={list(NP)} by the reference (flang build of "
                           f"v1.10/codes/main_v1.9.f90, {nr} MPI ranks, topology "
                           f"{tp[0]}x{tp[1]}x{tp[2]}); rate from its own CPU-time line for steps "
@@ -181,8 +224,8 @@ def main():
     var = formats.laminar_ic(n1, n2, n3)
 
     log(f"[bench] {args.precision}: warmup {args.warmup}, timed {args.steps} steps")
-    t, kms, kn, tau, v64, tiling = run_gpu(var, args.precision, args.steps, args.warmup,
-                                           profile=True, keep=not args.no_mixed)
+    t, kms, kn, tau, _, tiling = run_gpu(var, args.precision, args.steps, args.warmup,
+                                         profile=True)
     value = nodes * 2 * args.steps / t
     ms_step = 1000.0 * t / args.steps
     avg_launch_s = (kms / kn) / 1000.0 if kn else float("nan")
@@ -192,9 +235,16 @@ def main():
 
     mixed = None
     if not args.no_mixed and args.precision != "mixed":
-        tm, kmm, knm, taum, vmix, tilm = run_gpu(var, "mixed", args.steps, args.warmup,
-                                                 profile=True, keep=True)
+        tm, kmm, knm, taum, _, tilm = run_gpu(var, "mixed", args.steps, args.warmup,
+                                              profile=True)
         vm = nodes * 2 * args.steps / tm
+        mix_launch_s = (kmm / knm) / 1000.0 if knm else float("nan")
+        ach_m = ALG_BYTES_PER_NODE_STAGE_MIXED * nodes / mix_launch_s / 1e9
+        log(f"[bench] mixed: {1000 * tm / args.steps:.4f} ms/step, ratio {vm / value:.3f}, "
+            f"stage kernel {1000 * mix_launch_s:.4f} ms")
+        log(f"[bench] mixed vs fp64: {ERR_STEPS} steps each from the same start")
+        _, _, _, tau64, v64, _ = run_gpu(var, "fp64", ERR_STEPS, 0, profile=False, keep=True)
+        _, _, _, taum1k, vmix, _ = run_gpu(var, "mixed", ERR_STEPS, 0, profile=False, keep=True)
         names = ("rho", "u", "v", "w", "E", "T", "p")
         err = {nm: float(np.linalg.norm(vmix[c] - v64[c]) / np.linalg.norm(v64[c]))
                for c, nm in enumerate(names)}
@@ -202,13 +252,21 @@ def main():
         ubm = vmix[1, :, 1:-1, 1:-1].mean(axis=(1, 2))
         mixed = {"value": vm, "ms_per_step": 1000.0 * tm / args.steps,
                  "ratio_vs_fp64": vm / value,
-                 "stage_kernel_ms": (kmm / knm) if knm else None,
+                 "stage_kernel_ms": 1000 * mix_launch_s,
                  "tiling": tilm,
-                 "rel_l2_vs_fp64_after_steps": err,
+                 "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": ach_m / HBM_PEAK_GBS,
+                              "traffic": traffic_from_profiles("mixed"),
+                              "alg_bytes_per_node_stage": ALG_BYTES_PER_NODE_STAGE_MIXED,
+                              "kernel": "k_stage_fused<float>",
+                              "avg_launch_ms": 1000 * mix_launch_s},
+                 "error_steps": ERR_STEPS,
+                 "rel_l2_vs_fp64": err,
                  "mean_u_profile_rel_err": float(np.abs(ubm - ub64).max() / np.abs(ub64).max()),
-                 "tau_wall": [float(x) for x in taum], "tau_wall_fp64": [float(x) for x in tau]}
-        del vmix
-        log(f"[bench] mixed: {1000 * tm / args.steps:.4f} ms/step, ratio {vm / value:.3f}")
+                 "tau_wall": [float(x) for x in taum1k],
+                 "tau_wall_fp64": [float(x) for x in tau64]}
+        del vmix, v64
+        log(f"[bench] mixed: mean-u error {mixed['mean_u_profile_rel_err']:.3e} after {ERR_STEPS} steps")
 
     ens = None
     if not args.no_ensemble:
@@ -245,6 +303,8 @@ def main():
             "workload": f"channel DNS time step, {n1}x{n3}x{n2} (reference nP=[{n1},{n2},{n3}]), "
                         "Ma 1.5, Re 3000, RK2, hybrid central/QUICK, seeded perturbed laminar start",
             "nP": list(NP), "dt": DT, "istat": ISTAT, "precision": args.precision,
+            "cells": {"counted": nodes, "updated": (n1 - 2) * (n2 - 2) * (n3 - 2),
+                      "rule": "value and roofline count n1*n2*n3 nodes per stage"},
             "tiling": tiling,
             "mixed": mixed,
             "ensemble": ens,
@@ -257,6 +317,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(args.precision),
             "kernel": "k_stage_fused" if args.precision != "fp64-strict" else "k_rhs_strict",
+            "alg_bytes_per_node_stage": ALG_BYTES_PER_NODE_STAGE,
             "avg_launch_ms": 1000 * avg_launch_s,
         },
         "cpu_baseline": cpu,

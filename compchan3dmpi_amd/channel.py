@@ -25,7 +25,7 @@ PRECISIONS = {"fp64-strict": FP64_STRICT, "fp64": FP64, "mixed": MIXED}
 
 # Every function the C ABI exports (include/cc_abi.h).
 ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
-               "cc_blown", "cc_profile", "cc_tiling", "cc_destroy", "cc_last_error",
+               "cc_blown", "cc_profile", "cc_tiling", "cc_set_tiling", "cc_destroy", "cc_last_error",
                "cc_abi_version",
                "cc_batch_create", "cc_batch_member", "cc_batch_step", "cc_batch_tune",
                "cc_batch_destroy", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
@@ -93,6 +93,7 @@ def load_library():
     L.cc_blown.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.cc_profile.argtypes = [C.c_void_p, C.c_int, _dp]
     L.cc_tiling.argtypes = [C.c_void_p, C.c_void_p]
+    L.cc_set_tiling.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.cc_destroy.argtypes = [C.c_void_p]
     L.cc_destroy.restype = None
     L.cc_batch_create.argtypes = [C.POINTER(cc_params), C.c_int, C.POINTER(C.c_void_p)]
@@ -125,7 +126,8 @@ def load_library():
     L.cc_pp_qcriterion.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.cc_last_error.restype = C.c_char_p
     for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
-              "cc_blown", "cc_profile", "cc_tiling", "cc_abi_version", "cc_batch_create",
+              "cc_blown", "cc_profile", "cc_tiling", "cc_set_tiling", "cc_abi_version",
+              "cc_batch_create",
               "cc_batch_step",
               "cc_batch_tune", "cc_mlp_nparams", "cc_mlp_create", "cc_mlp_set_params",
               "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad", "cc_mlp_train",
@@ -235,6 +237,12 @@ class Channel:
         _check(self.L, self.L.cc_tiling(self.h, out.ctypes.data_as(C.c_void_p)))
         return {"tile": (int(out[0]), int(out[1])), "kchunk": int(out[2]), "blocks": int(out[3]),
                 "cus": int(out[4]), "blocks_per_cu": int(out[5])}
+
+    def set_tiling(self, kchunk, variant=-1):
+        """Force kchunk planes per block (and, mixed only, a tile shape
+        variant 0..3); kchunk <= 0 restores the automatic choice."""
+        _check(self.L, self.L.cc_set_tiling(self.h, int(variant), int(kchunk)))
+        return self.tiling()
 
     def filter_samples(self):
         """4x4x4 box-filtered features (N,16) and subgrid targets (N,16) of

@@ -517,6 +517,26 @@ int cc_tiling(void* handle, int* out) {
   return 0;
 }
 
+int cc_set_tiling(void* handle, int variant, int kchunk) {
+  Chan* c = (Chan*)handle;
+  if (!c) return fail(CC_EINVAL, "null handle");
+  if (c->prm.precision == CC_FP64_STRICT)
+    return fail(CC_EINVAL, "the reference-order path has no tiling");
+  if (c->in_batch) return fail(CC_EINVAL, "a batch member takes the batch's tiling");
+  const int mixed = c->prm.precision == CC_MIXED;
+  if (variant < -1 || variant > (mixed ? 3 : 0))
+    return fail(CC_EINVAL, mixed ? "tile variant must be -1..3" : "FP64 has tile variant 0 only");
+  if (kchunk <= 0) {
+    c->variant = choose_tiling(c->inst, mixed, &c->tile_ncu, &c->tile_bpc);
+  } else {
+    const int v = variant < 0 ? c->variant : variant;
+    set_tiling(c->inst, v, kchunk);
+    c->variant = v;
+  }
+  c->kchunk = c->inst.kchunk;
+  return 0;
+}
+
 int cc_set_closure(void* handle, void* mlp, const float* norm) {
   Chan* c = (Chan*)handle;
   if (!c) return fail(CC_EINVAL, "null handle");
@@ -583,7 +603,6 @@ int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
   HIPCHK(hipStreamSynchronize(c->s));
   return 0;
 }
-
 
 int cc_set_wall(void* handle, int wall, const cc_wall* w, const double* x1, const double* x2) {
   Chan* c = (Chan*)handle;
@@ -932,6 +951,13 @@ int cc_batch_tune(void* batch, int trial_steps, double* out) {
       snaps[i].n.push_back(pr.second);
     }
   }
+  // the first blown node each member has recorded: a trial's blow-up
+  // must not leak into a later cc_blown report
+  std::vector<unsigned long long> blow0(b->m.size());
+  for (size_t i = 0; i < b->m.size(); i++)
+    HIPCHK(hipMemcpyAsync(&blow0[i], b->m[i]->blow, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, b->s));
+  HIPCHK(hipStreamSynchronize(b->s));
   int step0 = 0;
   for (Chan* c : b->m) step0 = std::max(step0, c->nstep);
   std::vector<int> variants = b->mixed ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0};
@@ -963,7 +989,7 @@ int cc_batch_tune(void* batch, int trial_steps, double* out) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  // restore the state and the halt flags the trials may have set
+  // restore the state and the halt and blow-up flags the trials may have set
   for (size_t i = 0; i < b->m.size(); i++) {
     std::vector<std::pair<double*, size_t>> v;
     bufs(b->m[i], v);
@@ -974,6 +1000,8 @@ int cc_batch_tune(void* batch, int trial_steps, double* out) {
     Chan* c = b->m[i];
     const int h = c->blown_step;
     HIPCHK(hipMemcpyAsync(c->halt, &h, sizeof(int), hipMemcpyHostToDevice, b->s));
+    HIPCHK(hipMemcpyAsync(c->blow, &blow0[i], sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          b->s));
     HIPCHK(hipStreamSynchronize(b->s));
     for (double* d : snaps[i].dst) (void)hipFree(d);
   }

@@ -82,10 +82,14 @@ __device__ __forceinline__ float rcp<float>(float x) {
 // 2^(0.7 log2 x) = 2^n exp(f ln 2) with n = rint(.), |f| <= 1/2 and the
 // exponential by its Taylor series to degree 14.  Error <= 2 ulp over the
 // range a temperature can take, which the 1e-12 FP64 tolerance absorbs.
-// Float: the hardware log2/exp2 (v_log_f32, v_exp_f32).  The library's
-// log2f/exp2f wrap the same instructions in rescaling for denormal inputs
-// and results (12 more instructions), which a temperature never reaches.
-__device__ __forceinline__ double pow07d(double x) {
+// Outside x > 0 the series do not apply; the result is then what pow(x, 0.7)
+// gives there: 0 at x = 0, +inf at +inf, NaN for x < 0 and NaN, so that a
+// negative temperature propagates NaN exactly as the reference's T**0.7d0.
+// Float: the hardware log2/exp2 (v_log_f32, v_exp_f32), which give those
+// special values themselves.  The library's log2f/exp2f wrap the same
+// instructions in rescaling for denormal inputs and results (12 more
+// instructions), which a temperature never reaches.
+__device__ __forceinline__ double pow07d_series(double x) {
   int e;
   double m = frexp(x, &e);
   if (m < 0.70710678118654752440) {
@@ -125,6 +129,13 @@ __device__ __forceinline__ double pow07d(double x) {
   q = fma(q, g, 1.0);
   q = fma(q, g, 1.0);
   return ldexp(q, (int)n);
+}
+__device__ __forceinline__ double pow07d(double x) {
+  const double r = pow07d_series(x);
+  // finite positive x: the series; x = 0 and x = +inf return x itself
+  // (0**0.7 = 0, inf**0.7 = inf); x < 0 and NaN: NaN
+  if (x > 0.0 && x < __builtin_huge_val()) return r;
+  return (x == 0.0 || x > 0.0) ? x : __builtin_nan("");
 }
 
 template <typename T>

@@ -92,8 +92,12 @@ if (st == 0) then
 		prec = CC_FP64_STRICT
 	case ("mixed")
 		prec = CC_MIXED
-	case default
+	case ("fp64")
 		prec = CC_FP64
+	case default
+		write(6,'(3A)') "Error in CC_PRECISION: unknown mode '", trim(env), &
+			"' (fp64, fp64-strict or mixed)"
+		stop 1
 	end select
 endif
 

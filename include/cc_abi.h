@@ -107,6 +107,14 @@ int cc_profile(void* handle, int mode, double* out);
  * fp64-strict. */
 int cc_tiling(void* handle, int* out);
 
+/* Override that tiling (no counterpart in the reference): kchunk planes per
+ * block and, in the mixed mode, the tile shape (variant 0 = 32x16, 1 = 32x8,
+ * 2 = 16x16, 3 = 64x8; -1 keeps the current shape; FP64 has 0 only).
+ * kchunk <= 0 restores the automatic choice.  Results do not depend on the
+ * tiling: every choice computes the same values, which is what the override
+ * lets a test check.  CC_EINVAL for fp64-strict and batch members. */
+int cc_set_tiling(void* handle, int variant, int kchunk);
+
 void cc_destroy(void* handle);
 
 /* ---- batched ensembles (no counterpart in the reference, whose ensembles are

@@ -1,0 +1,236 @@
+"""The fused kernels' z-march at the tilings production runs use, configs 2
+and 3 at their full size, and the reference's behaviour when a stage drives
+the temperature negative.
+
+* Config 1 (64x32x48, 100 steps) with the planes per block forced to 2, 5
+  and 46 (cc_set_tiling): every tiling must give the same bits as the
+  automatic one (a node's arithmetic does not depend on which block marches
+  it), and FP64 must stay within 1e-12 of the oracle.  The automatic tiling
+  of a 256-CU device gives one plane per block at this grid, so these are
+  the cases that exercise the register windows, the LDS ping-pong and the
+  next-plane warming across many planes.
+* Config 2 (nP = [256,128,192], 48 planes per block in FP64): 4 steps from
+  the seeded start with statistics every 2 steps (forcing on from step 3),
+  against the oracle at 1e-12 in the conserved variables.
+* Config 3: the mixed mode against the FP64 GPU run after 1000 steps at the
+  same grid; the north-star bar is 1e-5 in mean velocity.
+* A state whose stage-1 predictor has T < 0 at one node: the reference's
+  mu = T**0.7d0 is NaN there (main_v1.9.f90:643), stage 2 spreads it, and its
+  check (rho < 0 or T < 0, :728-740) does not fire on NaN.  Every mode must
+  produce the oracle's NaN pattern and no halt.
+"""
+import numpy as np
+import pytest
+
+from compchan3dmpi_amd import formats
+from compchan3dmpi_amd.channel import Channel
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["rho", "u", "v", "w", "E", "T", "p"]
+
+
+def relL2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def conserved(var):
+    rho = var[0]
+    return np.stack([rho, rho * var[1], rho * var[2], rho * var[3], rho * var[4]])
+
+
+def banner(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def gpu_run(v0, n, steps, dt, istat, precision, kchunk=None):
+    ch = Channel(*n, dt=dt, istat=istat, precision=precision)
+    try:
+        if kchunk is not None:
+            t = ch.set_tiling(kchunk)
+            assert t["kchunk"] == kchunk, t
+        ch.upload(v0)
+        rc = ch.step(steps)
+        v, ns, _ = ch.download()
+        tau = ch.stats()[0]
+    finally:
+        ch.close()
+    return rc, v, ns, tau
+
+
+C1 = (64, 32, 48)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle():
+    v0 = formats.round_e22_15(formats.laminar_ic(*C1))
+    o = orc.Oracle(*C1, dt=0.0015, istat=10)
+    o.load(v0)
+    assert o.step(100) == 0
+    return v0, o.var(), o.stats()[0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c1_every_tiling_gives_the_same_bits(c1_oracle, precision):
+    v0, vo, tau_o = c1_oracle
+    banner(f"c1 100 steps {precision}, planes per block auto / 2 / 5 / 46")
+    rc, va, ns, tau_a = gpu_run(v0, C1, 100, 0.0015, 10, precision)
+    assert rc == 0 and ns == 100
+    for kc in (2, 5, 46):
+        rc, vk, ns, tau_k = gpu_run(v0, C1, 100, 0.0015, 10, precision, kchunk=kc)
+        assert rc == 0 and ns == 100
+        diff = int(np.count_nonzero(vk != va))
+        print(f"  kchunk {kc}: {diff} values differ from the automatic tiling", flush=True)
+        assert diff == 0, kc
+        assert np.array_equal(tau_k, tau_a)
+    e = relL2(conserved(va), conserved(vo))
+    print(f"  rel L2 conserved vs oracle = {e:.3e}", flush=True)
+    if precision == "fp64":
+        assert e <= 1e-12
+        assert np.allclose(tau_a, tau_o, rtol=1e-11, atol=1e-15)
+    else:
+        assert e <= 1e-4
+
+
+def test_c1_mixed_tile_shapes_agree(c1_oracle):
+    """The mixed kernel's four tile shapes, each at 3 planes per block.  Each
+    shape is a template instance of its own, whose FP32 code the compiler
+    contracts and orders on its own, so they agree to FP32 rounding, not
+    bit for bit; each stays within the mixed bar of the FP64 oracle."""
+    v0, vo, _ = c1_oracle
+    banner("c1 100 steps mixed, tile shapes 0..3 at 3 planes per block")
+    ref = None
+    for var in range(4):
+        ch = Channel(*C1, dt=0.0015, istat=10, precision="mixed")
+        try:
+            t = ch.set_tiling(3, var)
+            ch.upload(v0)
+            assert ch.step(100) == 0
+            v, _, _ = ch.download()
+        finally:
+            ch.close()
+        e = relL2(conserved(v), conserved(vo))
+        if ref is None:
+            ref = v
+        d = relL2(conserved(v), conserved(ref))
+        print(f"  tile {t['tile']}: vs oracle {e:.3e}, vs tile 32x16 {d:.3e}", flush=True)
+        assert e <= 1e-4
+        assert d <= 1e-6
+
+
+C2 = (256, 128, 192)
+DT2 = 9e-5  # CFL 0.5 of the seeded start, rounded down to the F6.4 grid (5 decimals)
+
+
+@pytest.fixture(scope="module")
+def c2_start():
+    v0 = formats.laminar_ic(*C2)
+    assert formats.cfl_dt(v0, 1.5) == DT2
+    return v0
+
+
+@pytest.mark.timeout(600)
+def test_c2_fp64_and_strict_match_the_oracle(c2_start):
+    """4 steps at nP=[256,128,192], istat = 2: the production FP64 tiling (48
+    planes per block on 256 CUs) and the reference-order path."""
+    v0 = c2_start
+    banner("c2 256x128x192 oracle, 4 steps istat=2")
+    o = orc.Oracle(*C2, dt=DT2, istat=2)
+    o.load(v0)
+    assert o.step(4) == 0
+    vo = o.var()
+    tau_o, s1o, s2o, cro = o.stats()
+    Co = conserved(vo)
+    del o
+    for prec in ("fp64", "fp64-strict"):
+        banner(f"c2 256x128x192 4 steps istat=2 {prec}")
+        ch = Channel(*C2, dt=DT2, istat=2, precision=prec)
+        try:
+            if prec == "fp64":
+                print(f"  tiling {ch.tiling()}", flush=True)
+            ch.upload(v0)
+            assert ch.step(4) == 0
+            vg, ns, _ = ch.download()
+            tau_g, s1g, s2g, crg = ch.stats()
+        finally:
+            ch.close()
+        assert ns == 4
+        e = relL2(conserved(vg), Co)
+        print(f"  rel L2 conserved = {e:.3e}", flush=True)
+        assert e <= 1e-12
+        for c in range(7):
+            assert relL2(vg[c], vo[c]) <= 1e-12, NAMES[c]
+        assert np.allclose(tau_g, tau_o, rtol=1e-11, atol=1e-15)
+        assert np.allclose(s1g, s1o, rtol=1e-12, atol=1e-15)
+        assert np.allclose(s2g, s2o, rtol=1e-12, atol=1e-15)
+        assert np.allclose(crg, cro, rtol=1e-11, atol=1e-14)
+        del vg
+
+
+@pytest.mark.timeout(300)
+def test_c3_mixed_against_fp64_after_1000_steps(c2_start):
+    """Config 3: per-field relative L2 error of the mixed mode against the
+    FP64 run after 1000 steps; mean streamwise velocity within 1e-5."""
+    v0 = c2_start
+    banner("c3 256x128x192 1000 steps mixed vs fp64")
+    res = {}
+    for prec in ("fp64", "mixed"):
+        rc, v, ns, tau = gpu_run(v0, C2, 1000, DT2, 10, prec)
+        assert rc == 0 and ns == 1000
+        res[prec] = (v, tau)
+    vf, tf = res["fp64"]
+    vm, tm = res["mixed"]
+    for c in range(7):
+        print(f"  {NAMES[c]:>3s} rel L2 = {relL2(vm[c], vf[c]):.3e}", flush=True)
+    ubar_f = vf[1, :, 1:-1, 1:-1].mean(axis=(1, 2))
+    ubar_m = vm[1, :, 1:-1, 1:-1].mean(axis=(1, 2))
+    e = np.abs(ubar_m - ubar_f).max() / np.abs(ubar_f).max()
+    print(f"  mean-u profile rel error = {e:.3e}; tau fp64 {tf}, mixed {tm}", flush=True)
+    assert e <= 1e-5
+    assert relL2(conserved(vm), conserved(vf)) <= 1e-5
+    assert np.allclose(tm, tf, rtol=1e-5)
+
+
+def stage1_negative_T_start():
+    """12x8x16: a velocity spike of 5 over a 3x2x3 block and T = 0.02 at one
+    node of it; with dt = 5e-3 the stage-1 predictor's T is negative at that
+    node while the state after the full step is not (found with the oracle)."""
+    v0 = formats.laminar_ic(12, 8, 16)
+    v0[1, 5:8, 3:5, 4:7] += 5.0
+    v0[5, 6, 4, 5] = 0.02
+    v0[6] = v0[0] * v0[5] / (1.4 * 1.5 ** 2)
+    v0[4] = v0[5] + 1.26 * 0.5 * (v0[1] ** 2 + v0[2] ** 2 + v0[3] ** 2)
+    return formats.round_e22_15(v0)
+
+
+@pytest.fixture(scope="module")
+def nan_case():
+    v0 = stage1_negative_T_start()
+    o = orc.Oracle(12, 8, 16, dt=5e-3, istat=100)
+    o.load(v0)
+    o.stage(1)
+    t1 = o.var()[5, 1:-1, 1:-1, 1:-1].min()
+    assert t1 < -1e-3, t1  # the case: T < 0 after stage 1
+    o = orc.Oracle(12, 8, 16, dt=5e-3, istat=100)
+    o.load(v0)
+    assert o.step(1) == 0  # NaN is not caught by rho < 0 or T < 0
+    vo = o.var()
+    mask = np.isnan(vo)
+    assert mask.any() and not mask.all()
+    assert np.nanmin(vo[5, 1:-1, 1:-1, 1:-1]) > 0.0
+    return v0, vo, mask
+
+
+@pytest.mark.parametrize("precision", ["fp64-strict", "fp64", "mixed"])
+def test_stage1_negative_temperature_propagates_nan(nan_case, precision):
+    v0, vo, mask = nan_case
+    banner(f"stage-1 T < 0 at one node, 1 step {precision}")
+    rc, vg, ns, _ = gpu_run(v0, (12, 8, 16), 1, 5e-3, 100, precision)
+    assert rc == 0 and ns == 1
+    gm = np.isnan(vg)
+    print(f"  NaN values: oracle {int(mask.sum())}, HIP {int(gm.sum())}", flush=True)
+    assert np.array_equal(gm, mask)
+    tol = 1e-4 if precision == "mixed" else 1e-12
+    f = ~mask
+    assert np.linalg.norm(vg[f] - vo[f]) <= tol * np.linalg.norm(vo[f])
