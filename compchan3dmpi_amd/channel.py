@@ -24,7 +24,8 @@ FP64_STRICT, FP64, MIXED = 0, 1, 2
 PRECISIONS = {"fp64-strict": FP64_STRICT, "fp64": FP64, "mixed": MIXED}
 
 # Every function the C ABI exports (include/cc_abi.h).
-ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
+ABI_SYMBOLS = ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_diag_noalloc",
+               "cc_download",
                "cc_blown", "cc_profile", "cc_tiling", "cc_set_tiling", "cc_destroy", "cc_last_error",
                "cc_abi_version",
                "cc_batch_create", "cc_batch_member", "cc_batch_step", "cc_batch_tune",
@@ -89,6 +90,7 @@ def load_library():
     L.cc_step.argtypes = [C.c_void_p, C.c_int]
     L.cc_stats.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.cc_diag.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+    L.cc_diag_noalloc.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
     L.cc_download.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.cc_blown.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.cc_profile.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -125,7 +127,8 @@ def load_library():
     L.cc_pp_fluctuation.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
     L.cc_pp_qcriterion.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.cc_last_error.restype = C.c_char_p
-    for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_download",
+    for f in ("cc_create", "cc_upload", "cc_step", "cc_stats", "cc_diag", "cc_diag_noalloc",
+              "cc_download",
               "cc_blown", "cc_profile", "cc_tiling", "cc_set_tiling", "cc_abi_version",
               "cc_batch_create",
               "cc_batch_step",
@@ -213,6 +216,15 @@ class Channel:
         out = np.empty(16)
         _check(self.L, self.L.cc_diag(self.h, np.ascontiguousarray(x3_file, dtype=np.float64),
                                       float(x2_end), out))
+        return out
+
+    def diag_noalloc(self, x3_file, x2_end):
+        """diag() through cc_diag_noalloc: no device allocation, partials in
+        the library's static per-GPU scratch (n3 <= 4096)."""
+        out = np.empty(16)
+        _check(self.L, self.L.cc_diag_noalloc(self.h,
+                                              np.ascontiguousarray(x3_file, dtype=np.float64),
+                                              float(x2_end), out))
         return out
 
     def download(self):

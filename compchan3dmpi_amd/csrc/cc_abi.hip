@@ -41,6 +41,7 @@ struct Chan {
   Phys ph{};
   int n1 = 0, n2 = 0, n3 = 0;
   hipStream_t s = nullptr;
+  int dev = 0;  // the device current at cc_create
   double *UA = nullptr, *UB = nullptr, *W = nullptr, *rw = nullptr, *pw = nullptr;
   double *uwall = nullptr, *vwall = nullptr;  // wall velocities [2][plane]
   double *twall = nullptr, *mwall = nullptr;  // wall T and mu = T^0.7 [2][plane]
@@ -259,10 +260,16 @@ int cc_create(const cc_params* p, void** handle) {
   if (p->Ri_b != 0.0 || p->slip_len != 0.0)
     if (p->precision == CC_FP64_STRICT)
       return fail(CC_EINVAL, "buoyancy and wall slip are not part of the reference-order path");
-  int ndev = 0;
+  int ndev = 0, dev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(CC_EHIP, "no HIP device");
+  if (hipGetDevice(&dev) != hipSuccess) return fail(CC_EHIP, "no current HIP device");
+  // the allocation-free entry points' static scratch on this device: its
+  // address, once per device (cc_diag_noalloc)
+  if (diag_scratch_init(dev) != 0)
+    return fail(CC_EHIP, "hipGetSymbolAddress of the static scratch failed");
   Chan* c = new Chan();
+  c->dev = dev;
   c->prm = *p;
   c->n1 = p->n1;
   c->n2 = p->n2;
@@ -587,10 +594,10 @@ int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
   if (!c->loaded) return fail(CC_ESTATE, "no state uploaded");
   const size_t full = (size_t)c->n1 * c->n2 * c->n3;
   if (!c->Vdev) HIPCHK(hipMalloc((void**)&c->Vdev, 7 * full * sizeof(double)));
-  if (!c->dbuf) HIPCHK(hipMalloc((void**)&c->dbuf, (1024 + 16 + c->n3) * sizeof(double)));
+  if (!c->dbuf) HIPCHK(hipMalloc((void**)&c->dbuf, (kDiagOffX3 + c->n3) * sizeof(double)));
   double* part = c->dbuf;
-  double* dout = c->dbuf + 1024;
-  double* x3f = c->dbuf + 1040;
+  double* dout = c->dbuf + kDiagOffOut;
+  double* x3f = c->dbuf + kDiagOffX3;
   HIPCHK(hipMemcpyAsync(x3f, x3_file, c->n3 * sizeof(double), hipMemcpyHostToDevice, c->s));
   const double* V = c->Vfile;
   if (!c->fresh) {
@@ -598,6 +605,31 @@ int cc_diag(void* handle, const double* x3_file, double x2_end, double* out) {
     V = c->Vdev;
   }
   launch_diag(c->n1, c->n2, c->n3, c->ph.dy, V, part, x3f, x2_end, dout, c->s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, 16 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_diag_noalloc(void* handle, const double* x3_file, double x2_end, double* out) {
+  Chan* c = (Chan*)handle;
+  if (!c || !x3_file || !out) return fail(CC_EINVAL, "null argument");
+  if (!c->loaded) return fail(CC_ESTATE, "no state uploaded");
+  if (c->n3 > kDiagScratchMaxN3)
+    return fail(CC_EINVAL, "cc_diag_noalloc: n3 = " + std::to_string(c->n3) +
+                               " exceeds its static scratch (n3 <= " +
+                               std::to_string(kDiagScratchMaxN3) + "); use cc_diag");
+  double* ptr = diag_scratch(c->dev);  // the device address, never the array's name
+  if (!ptr) return fail(CC_ESTATE, "no static scratch on this handle's device");
+  double* part = ptr;
+  double* dout = ptr + kDiagOffOut;
+  double* x3f = ptr + kDiagOffX3;
+  HIPCHK(hipMemcpyAsync(x3f, x3_file, c->n3 * sizeof(double), hipMemcpyHostToDevice, c->s));
+  if (c->fresh)  // before the first step: the uploaded file itself, as cc_diag
+    launch_diag(c->n1, c->n2, c->n3, c->ph.dy, c->Vfile, part, x3f, x2_end, dout, c->s);
+  else
+    launch_diag_state(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->Vfile, part, x3f,
+                      x2_end, dout, c->s);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, dout, 16 * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));

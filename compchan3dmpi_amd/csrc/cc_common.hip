@@ -2,6 +2,8 @@
 // the bulk forcing (wall shear and plane-mean density), shared by every
 // precision mode.  Compiled with -ffp-contract=off: the per-node primitive
 // recovery here must give the same bits as the stage kernels' own.
+#include <mutex>
+
 #include "cc_device.h"
 #include "cc_kernels.h"
 
@@ -210,22 +212,27 @@ __global__ void k_wall_init(Geom g, Phys ph, int n1, int n2, const double* __res
 }
 
 // ---- download: the reference's var(n1,n2,n3,7) with every halo value ----
-__global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __restrict__ U,
-                           WallP pw, const double* __restrict__ rw, const double* __restrict__ V,
-                           double* __restrict__ Vout) {
-  const long long total = (long long)n1 * n2 * g.n3;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const int i = (int)(t % n1);
-  const int j = (int)((t / n1) % n2);
-  const int k = (int)(t / ((long long)n1 * n2));
+// var(i,j,k,1..7) (Fortran 0-based i,j,k) as the reference holds it, from
+// the device state: periodic images of the unique nodes, the stale corner
+// columns of the interior planes (the file's values in V), the wall values
+// and, on a wall, its duplicate ring.
+__device__ __forceinline__ Prim node_var(const Geom& g, const Phys& ph, int n1, int n2,
+                                         const double* __restrict__ U, const WallP& pw,
+                                         const double* __restrict__ rw,
+                                         const double* __restrict__ V, int i, int j, int k) {
   const int n3 = g.n3;
   const bool ei = (i == 0 || i == n1 - 1), ej = (j == 0 || j == n2 - 1);
   const bool wall = (k == 0 || k == n3 - 1);
   Prim P;
   if (!wall && ei && ej) {  // stale corner column
-    for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, n3, i, j, k, c)] = V[fidx(n1, n2, n3, i, j, k, c)];
-    return;
+    P.rho = V[fidx(n1, n2, n3, i, j, k, 0)];
+    P.u = V[fidx(n1, n2, n3, i, j, k, 1)];
+    P.v = V[fidx(n1, n2, n3, i, j, k, 2)];
+    P.w = V[fidx(n1, n2, n3, i, j, k, 3)];
+    P.E = V[fidx(n1, n2, n3, i, j, k, 4)];
+    P.T = V[fidx(n1, n2, n3, i, j, k, 5)];
+    P.p = V[fidx(n1, n2, n3, i, j, k, 6)];
+    return P;
   }
   const int top = k == n3 - 1;
   if (wall && (ei || ej) && ((pw.wflags >> top) & 1)) {
@@ -248,8 +255,21 @@ __global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __rest
       P = prim_of(ph, U[n], U[F + n], U[2 * F + n], U[3 * F + n], U[4 * F + n]);
     }
   }
+  return P;
+}
+
+__global__ void k_download(Geom g, Phys ph, int n1, int n2, const double* __restrict__ U,
+                           WallP pw, const double* __restrict__ rw, const double* __restrict__ V,
+                           double* __restrict__ Vout) {
+  const long long total = (long long)n1 * n2 * g.n3;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int i = (int)(t % n1);
+  const int j = (int)((t / n1) % n2);
+  const int k = (int)(t / ((long long)n1 * n2));
+  const Prim P = node_var(g, ph, n1, n2, U, pw, rw, V, i, j, k);
   const double v[7] = {P.rho, P.u, P.v, P.w, P.E, P.T, P.p};
-  for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, n3, i, j, k, c)] = v[c];
+  for (int c = 0; c < 7; c++) Vout[fidx(n1, n2, g.n3, i, j, k, c)] = v[c];
 }
 
 // ---- deterministic block reduction ---------------------------------------
@@ -396,16 +416,61 @@ __global__ void k_planestats_final(int n1, int n2, int n3, const double* __restr
   }
 }
 
-// ---- bulk diagnostics on the reconstructed var (main_v1.9.f90:799-906) ----
+// ---- bulk diagnostics (main_v1.9.f90:799-906) --------------------------------
+// The diagnostics read var(i,j,k,c) of every node.  Two sources give the same
+// values: the reconstructed array (VarArray, what cc_download returns) or the
+// device state directly (VarState: node_var per node, no 7-field copy).
+struct VarArray {
+  const double* V;
+  long long total;
+  __device__ void uvw(long long t, double& u, double& v, double& w) const {
+    u = V[total + t];
+    v = V[2 * total + t];
+    w = V[3 * total + t];
+  }
+  __device__ void ru(long long t, double& r, double& u) const {
+    r = V[t];
+    u = V[total + t];
+  }
+};
+struct VarState {
+  Geom g;
+  Phys ph;
+  int n1, n2;
+  const double* U;
+  WallP pw;
+  const double* rw;
+  const double* V;
+  __device__ Prim at(long long t) const {
+    const int i = (int)(t % n1);
+    const int j = (int)((t / n1) % n2);
+    const int k = (int)(t / ((long long)n1 * n2));
+    return node_var(g, ph, n1, n2, U, pw, rw, V, i, j, k);
+  }
+  __device__ void uvw(long long t, double& u, double& v, double& w) const {
+    const Prim P = at(t);
+    u = P.u;
+    v = P.v;
+    w = P.w;
+  }
+  __device__ void ru(long long t, double& r, double& u) const {
+    const Prim P = at(t);
+    r = P.rho;
+    u = P.u;
+  }
+};
+
 // Pass 1: per-block partial sums of u^2+v^2+w^2 over every node (halo values
 // included, as the reference's all-node loop).
-__global__ void __launch_bounds__(256) k_ke_partial(long long total, const double* __restrict__ V,
+template <typename A>
+__global__ void __launch_bounds__(256) k_ke_partial(long long total, A var,
                                                     double* __restrict__ part) {
   __shared__ double lds[4];
   double acc[1] = {0.0};
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
-    const double u = V[total + t], v = V[2 * total + t], w = V[3 * total + t];
+    double u, v, w;
+    var.uvw(t, u, v, w);
     acc[0] += u * u + v * v + w * w;
   }
   block_sum<1>(acc, lds);
@@ -415,13 +480,12 @@ __global__ void __launch_bounds__(256) k_ke_partial(long long total, const doubl
 // Pass 2 (one block): KE from the partials in a fixed order, then the three
 // bulk plane integrals for each of the five x-planes with the trapezoidal
 // quarter-sum of main_v1.9.f90:827-834.
-__global__ void __launch_bounds__(256) k_diag_final(int n1, int n2, int n3, double dy,
-                                                    const double* __restrict__ V,
+template <typename A>
+__global__ void __launch_bounds__(256) k_diag_final(int n1, int n2, int n3, double dy, A var,
                                                     const double* __restrict__ part, int npart,
                                                     const double* __restrict__ x3f, double x2e,
                                                     double* __restrict__ out) {
   __shared__ double lds[4];
-  const long long total = (long long)n1 * n2 * n3;
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int b = 0; b < npart; b++) s += part[b];
@@ -441,9 +505,11 @@ __global__ void __launch_bounds__(256) k_diag_final(int n1, int n2, int n3, doub
           const double dAyz = dy * (x3f[k + 1] - x3f[k]);
           auto val = [&](int jj, int kk) {
             const long long n = ((long long)kk * n2 + jj) * n1 + i;
-            if (q == 0) return V[n];
-            if (q == 1) return V[total + n];
-            return V[n] * V[total + n];
+            double r, u;
+            var.ru(n, r, u);
+            if (q == 0) return r;
+            if (q == 1) return u;
+            return r * u;
           };
           acc[0] += 0.25 * (val(j, k) + val(j + 1, k) + val(j, k + 1) + val(j + 1, k + 1)) * dAyz;
         }
@@ -457,9 +523,56 @@ __global__ void __launch_bounds__(256) k_diag_final(int n1, int n2, int n3, doub
 void launch_diag(int n1, int n2, int n3, double dy, const double* Vdev, double* part,
                  const double* x3f, double x2e, double* out, hipStream_t s) {
   const long long total = (long long)n1 * n2 * n3;
-  hipLaunchKernelGGL(k_ke_partial, dim3(1024), dim3(256), 0, s, total, Vdev, part);
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, n1, n2, n3, dy, Vdev, part, 1024,
-                     x3f, x2e, out);
+  const VarArray A{Vdev, total};
+  hipLaunchKernelGGL(k_ke_partial<VarArray>, dim3(kDiagParts), dim3(256), 0, s, total, A, part);
+  hipLaunchKernelGGL(k_diag_final<VarArray>, dim3(1), dim3(256), 0, s, n1, n2, n3, dy, A, part,
+                     kDiagParts, x3f, x2e, out);
+}
+
+// ---- the allocation-free diagnostics' scratch --------------------------------
+// One static device array per GPU the library runs on, held until the process
+// ends: kDiagParts partial sums, the 16 results and the file's x3 (n3 <=
+// kDiagScratchMaxN3).  Every call on a GPU shares it; each call writes every
+// element it reads, so no clearing is needed, and calls that overlap on one
+// GPU race.  In host code the array's name is a host-side stand-in, not a
+// device address: only the pointer hipGetSymbolAddress returns for a device,
+// with that device current, is passed to kernels and copies.
+static __device__ __align__(256) double scratch[kDiagScratchSize];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int diag_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess) return -1;
+  if (cur != dev && hipSetDevice(dev) != hipSuccess) return -1;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p) return -1;
+  g_scratch_ptr[dev] = (double*)p;
+  return 0;
+}
+
+double* diag_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
+
+void launch_diag_state(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
+                       const double* rw, const double* Vfile, double* part, const double* x3f,
+                       double x2e, double* out, hipStream_t s) {
+  const long long total = (long long)n1 * n2 * g.n3;
+  const VarState A{g, ph, n1, n2, U, pw, rw, Vfile};
+  hipLaunchKernelGGL(k_ke_partial<VarState>, dim3(kDiagParts), dim3(256), 0, s, total, A, part);
+  hipLaunchKernelGGL(k_diag_final<VarState>, dim3(1), dim3(256), 0, s, n1, n2, g.n3, ph.dy, A,
+                     part, kDiagParts, x3f, x2e, out);
 }
 
 void launch_upload(const Geom& g, const Phys& ph, int n1, int n2, const double* Vfull, double* UA,

@@ -121,6 +121,25 @@ void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double
 // x3f (n3) and x2e are the file's mesh values.
 void launch_diag(int n1, int n2, int n3, double dy, const double* Vdev, double* part,
                  const double* x3f, double x2e, double* out, hipStream_t s);
+// The same diagnostics read from the device state (node by node, no
+// reconstructed copy), with part, out and x3f in the caller's scratch.
+void launch_diag_state(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
+                       const double* rw, const double* Vfile, double* part, const double* x3f,
+                       double x2e, double* out, hipStream_t s);
+// Partial sums of the diagnostics' first pass, and the layout of the static
+// scratch of the allocation-free diagnostics: [kDiagParts] partials, [16]
+// results, [kDiagScratchMaxN3] x3 of the file.
+constexpr int kDiagParts = 1024;
+constexpr int kDiagScratchMaxN3 = 4096;
+constexpr int kDiagOffOut = kDiagParts;
+constexpr int kDiagOffX3 = kDiagParts + 16;
+constexpr int kDiagScratchSize = kDiagOffX3 + kDiagScratchMaxN3;
+constexpr int kMaxDevices = 64;
+// Once per device, with that device made current: the device address of the
+// static scratch (hipGetSymbolAddress); 0 or -1.  diag_scratch: that address,
+// or null before the set-up.
+int diag_scratch_init(int dev);
+double* diag_scratch(int dev);
 // tau[0], tau[1] -> force[0], force[1]; plane statistics -> stats[23][n3]
 // and force[2 + kk] = mean density of plane kk.
 // part: scratch of stats_scratch_size(n3) doubles

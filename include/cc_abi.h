@@ -84,6 +84,17 @@ int cc_stats(void* handle, double* tau, double* stat1, double* stat2, double* co
  * them (:821-830). */
 int cc_diag(void* handle, const double* x3_file, double x2_end, double* out);
 
+/* The same diagnostics with no device allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  cc_diag
+ * allocates, on its first call, a copy of var (7 n1 n2 n3 doubles) and its
+ * partial sums; this variant reads the device state node by node and keeps
+ * its partial sums, results and x3_file in a static device array of 41 KB
+ * per GPU, held until the process ends.  Limit: n3 <= 4096 (CC_EINVAL
+ * beyond).  Calls on one GPU share that array: calls that overlap on one GPU
+ * race.  Same results as cc_diag. */
+int cc_diag_noalloc(void* handle, const double* x3_file, double x2_end, double* out);
+
 /* Current state as var(n1,n2,n3,7), every halo value as the reference holds
  * it (periodic images, stale corners and wall rings). */
 int cc_download(void* handle, double* var, int* nstep, double* time);
