@@ -28,7 +28,7 @@ Fixtures (all inputs are regenerated from the seed by the tests):
                           plotInfo.dat, sha256 of the 3D.dat it read, of
                           fluc.dat and qInvariant.dat, and Q on plane k=8
 
-    python tests/golden/make_golden.py studies    # only the study fixtures
+    python tests/golden/make_golden.py studies [NAME ...]   # only the study fixtures
     python tests/golden/make_golden.py post       # only the post-processing one
 """
 from __future__ import annotations
@@ -52,14 +52,15 @@ def ic(n1, n2, n3):
     return formats.round_e22_15(formats.laminar_ic(n1, n2, n3))
 
 
-STUDIES = ("s1", "s3", "s4", "s5", "rd1", "rd3", "g1", "gst2", "gst3", "gst5", "st1", "st3")
+STUDIES = ("s1", "s2", "s3", "s4", "s5", "s6", "rd1", "rd2", "rd3", "g1", "gst1", "gst2", "gst3",
+           "gst4", "gst5", "st1", "st2", "st3", "st4")
 STUDY_GRID = (24, 20, 16)
 
 
-def studies():
+def studies(names=STUDIES):
     n1, n2, n3 = STUDY_GRID
     v = ic(n1, n2, n3)
-    for st in STUDIES:
+    for st in names:
         r = ref_run.run(v, formats.Params(dt=0.001, istat=5), 10, 10, study=st)
         var = np.ascontiguousarray(r["var"])
         np.savez(
@@ -107,8 +108,8 @@ def post():
 
 
 def main():
-    if sys.argv[1:] == ["studies"]:
-        studies()
+    if sys.argv[1:2] == ["studies"]:
+        studies(tuple(sys.argv[2:]) or STUDIES)
         return
     if sys.argv[1:] == ["post"]:
         post()

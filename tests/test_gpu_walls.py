@@ -12,7 +12,8 @@ import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-STUDIES = ("s1", "s3", "s4", "s5", "rd1", "rd3", "g1", "gst2", "gst3", "gst5", "st1", "st3")
+STUDIES = ("s1", "s2", "s3", "s4", "s5", "s6", "rd1", "rd2", "rd3", "g1", "gst1", "gst2", "gst3",
+           "gst4", "gst5", "st1", "st2", "st3", "st4")
 
 
 def banner(msg):
