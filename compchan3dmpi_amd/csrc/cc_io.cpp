@@ -446,8 +446,11 @@ int cc_read_solution(const char* path, int n1, int n2, int n3, double* x1, doubl
   if (!path) return io_fail(CC_EINVAL, "null path");
   FILE* f = fopen(path, "r");
   if (!f) return io_fail(CC_EINVAL, std::string("cannot open ") + path);
-  char line[512] = {0};
+  char line[512] = {0}, vars[512] = {0};
   const bool ok = fgets(line, sizeof line, f) != nullptr;
+  // the halo-corrected solver's files add an integer "Pattern" column after
+  // the ten values (v1.10/codes/main_v1.9.f90:777,788)
+  const bool has_pattern = fgets(vars, sizeof vars, f) != nullptr && strstr(vars, "Pattern");
   fclose(f);
   if (!ok || strlen(line) < 37) return io_fail(CC_EINVAL, std::string(path) + ": bad title line");
   // (8X,F20.10,I9,1X), main_v1.9.f90:227-231
@@ -458,7 +461,7 @@ int cc_read_solution(const char* path, int n1, int n2, int n3, double* x1, doubl
     b[9] = 0;
     *nstep = atoi(b);
   }
-  return cc_read_points(path, 3, n1, n2, n3, 7, 0, x1, x2, x3, var, nthreads);
+  return cc_read_points(path, 3, n1, n2, n3, 7, has_pattern ? 1 : 0, x1, x2, x3, var, nthreads);
 }
 
 int cc_solution_dims(const char* path, int* n1, int* n2, int* n3) {

@@ -233,6 +233,70 @@ def write_3d(path: str, time: float, nstep: int, x1, x2, x3, var: np.ndarray,
         f.write("\n".join(out) + "\n")
 
 
+def decompose(n, topo):
+    """((bs1, bs2, bs3), (es1, es2, es3)) of every rank of an MPI topology over
+    an n1 x n2 x n3 grid, 1-based and inclusive, in rank order: the reference's
+    split (main_v1.9.f90:110-145; along each axis the first topo - mod(n, topo)
+    ranks take n // topo nodes, the others one more) and MPI's row-major
+    numbering of the Cartesian communicator it creates without reordering
+    (:90)."""
+    axes = []
+    for d in range(3):
+        sz, cut, first, b = n[d] // topo[d], topo[d] - n[d] % topo[d], 1, []
+        for r in range(topo[d]):
+            if r == cut:
+                sz += 1
+            b.append((first, first + sz - 1))
+            first += sz
+        axes.append(b)
+    return [((axes[0][a][0], axes[1][b][0], axes[2][c][0]),
+             (axes[0][a][1], axes[1][b][1], axes[2][c][1]))
+            for a in range(topo[0]) for b in range(topo[1]) for c in range(topo[2])]
+
+
+def dominfo_text(n, blocks) -> str:
+    """domInfo.txt as the reference writes it (main_v1.9.f90:801-811): nP
+    (3(i4)), the rank count (i3), each rank's bounds (6(i4,1x)); the 1X that
+    ends a record writes nothing."""
+    lines = [f"{n[0]:4d}{n[1]:4d}{n[2]:4d}", f"{len(blocks):3d}"]
+    for bs, es in blocks:
+        lines.append(" ".join(f"{v:4d}" for v in (bs[0], es[0], bs[1], es[1], bs[2], es[2])))
+    return "\n".join(lines) + "\n"
+
+
+def write_rank_files(outdir, time, nstep, x1, x2, x3, var, topo, suffix=""):
+    """The per-rank solution files 3Dnnn<suffix>.dat and domInfo.txt that the
+    reference's ranks write for this topology (main_v1.9.f90:769-811)."""
+    _, n3, n2, n1 = var.shape
+    blocks = decompose((n1, n2, n3), topo)
+    for r, (bs, es) in enumerate(blocks):
+        sl = (slice(None), slice(bs[2] - 1, es[2]), slice(bs[1] - 1, es[1]), slice(bs[0] - 1, es[0]))
+        write_3d(os.path.join(outdir, f"3D{r + 1:03d}{suffix}.dat"), time, nstep,
+                 x1[bs[0] - 1:es[0]], x2[bs[1] - 1:es[1]], x3[bs[2] - 1:es[2]], var[sl])
+    with open(os.path.join(outdir, "domInfo.txt"), "w") as f:
+        f.write(dominfo_text((n1, n2, n3), blocks))
+    return blocks
+
+
+def read_rank_files(outdir, suffix=""):
+    """(time, nstep, x1, x2, x3, var) assembled from the per-rank files listed
+    by outdir/domInfo.txt (the reference's combine program,
+    v1.9/codes/combine.f90:15-58)."""
+    with open(os.path.join(outdir, "domInfo.txt")) as f:
+        rows = [l for l in f.read().split("\n") if l.strip()]
+    n1, n2, n3 = int(rows[0][0:4]), int(rows[0][4:8]), int(rows[0][8:12])
+    nproc = int(rows[1])
+    var = np.zeros((7, n3, n2, n1))
+    x1, x2, x3 = np.zeros(n1), np.zeros(n2), np.zeros(n3)
+    t = ns = None
+    for r in range(nproc):
+        b = [int(v) for v in rows[2 + r].split()]
+        t, ns, y1, y2, y3, v = read_3d(os.path.join(outdir, f"3D{r + 1:03d}{suffix}.dat"))
+        var[:, b[4] - 1:b[5], b[2] - 1:b[3], b[0] - 1:b[1]] = v
+        x1[b[0] - 1:b[1]], x2[b[2] - 1:b[3]], x3[b[4] - 1:b[5]] = y1, y2, y3
+    return t, ns, x1, x2, x3, var
+
+
 _HDR = re.compile(r'TITLE\s*="\s*(\S+)\s+(\d+)\s*"')
 
 

@@ -242,7 +242,8 @@ int cc_write_solution(const char* path, double time, int nstep, int n1, int n2, 
                       const double* x1, const double* x2, const double* x3, const double* var,
                       const int* pattern, int nthreads);
 /* Reads one (replaces main_v1.9.f90:210-251): time, nstep from the title
- * line, x1, x2, x3 and var (n1,n2,n3,7); any output pointer may be NULL. */
+ * line, x1, x2, x3 and var (n1,n2,n3,7); any output pointer may be NULL.
+ * A "Pattern" column (the halo-corrected solver's files) is skipped. */
 int cc_read_solution(const char* path, int n1, int n2, int n3, double* x1, double* x2,
                      double* x3, double* var, double* time, int* nstep, int nthreads);
 /* Grid of a solution file from its ZONE k=,j=,i= line. */

@@ -23,7 +23,10 @@
 # instrument the output only: the 3D solution file is written every NWRITE
 # steps instead of every 100 (main_v1.9.f90:769) and with ES25.17 instead of
 # E22.15 (main_v1.9.f90:788) so that a file holds every bit of the state.
-# The numerics are untouched.  MPI is the image's MPICH 3.3.2; its Fortran
+# The numerics are untouched.  With REF_FMT=E22.15 in the environment the
+# files keep the reference's own E22.15 format (output directory suffix
+# _e22), so that the build can restart (resOp=2) from the files it wrote,
+# whose reader expects that format.  MPI is the image's MPICH 3.3.2; its Fortran
 # module was built by gfortran and cannot be read by flang, so `use mpi` is
 # satisfied by compiling MPICH's own mpif.h into a module (what MPICH's
 # mpi module is made from); at one rank every exchange is a self-copy.
@@ -31,7 +34,9 @@ set -euo pipefail
 N1=${1:?N1}; N2=${2:?N2}; N3=${3:?N3}; MAXSTEP=${4:?MAXSTEP}; NWRITE=${5:-100}; STUDY=${6:-}
 REF=/root/reference
 HERE=$(cd "$(dirname "$0")" && pwd)
-OUT="$HERE/_ref/n${N1}x${N2}x${N3}_s${MAXSTEP}_w${NWRITE}${STUDY:+_$STUDY}"
+FMTSUF=""
+[ "${REF_FMT:-}" = "E22.15" ] && FMTSUF="_e22"
+OUT="$HERE/_ref/n${N1}x${N2}x${N3}_s${MAXSTEP}_w${NWRITE}${STUDY:+_$STUDY}${FMTSUF}"
 MAINSRC="$REF/v1.10/codes/main_v1.9.f90"
 COMSRC="$REF/v1.9/codes/comdata.f90"
 SPLICE=""
@@ -76,13 +81,15 @@ fi
 
 # output cadence (every NWRITE steps, alternating on nstep/100 as v1.9) and
 # full-precision solution files
+WFMT="ES25.17"
+[ -n "$FMTSUF" ] && WFMT="E22.15"
 sed -e "s/if(mod(nstep,100).eq.0) then/if(mod(nstep,${NWRITE}).eq.0) then/" \
     -e "s/if(mod(nstep,1000).eq.0) then/if(mod(nstep,${NWRITE}).eq.0) then/" \
     -e "s/if(mod(nstep\/1000,2).eq.0) then/if(mod(nstep\/100,2).eq.0) then/" \
-    -e "s/write(10,'(3(1X,F10.6),7(1X,E22.15),1X,I1)')/write(10,'(3(1X,F10.6),7(1X,ES25.17),1X,I1)')/" \
+    -e "s/write(10,'(3(1X,F10.6),7(1X,E22.15),1X,I1)')/write(10,'(3(1X,F10.6),7(1X,${WFMT}),1X,I1)')/" \
     main_src.f90 > main_ref.f90
 grep -q "mod(nstep,${NWRITE}).eq.0" main_ref.f90 || { echo "cadence substitution failed" >&2; exit 4; }
-grep -q "ES25.17" main_ref.f90 || { echo "format substitution failed" >&2; exit 4; }
+grep -q "7(1X,${WFMT}),1X,I1" main_ref.f90 || { echo "format substitution failed" >&2; exit 4; }
 
 printf '      module mpi\n      include '"'"'mpif.h'"'"'\n      end module mpi\n' > mpi_mod.f
 

@@ -20,6 +20,14 @@ Fixtures (all inputs are regenerated from the seed by the tests):
                           L2, Linf, the bottom wall plane (7 fields),
                           tauWall and moment1st rows
 
+  restart_24x20x16_topo212_s20_s40.npz
+                          the reference on 4 MPI ranks (topology 2x1x2), 20
+                          steps from the seeded start, then restarted from its
+                          per-rank files to step 40 (dt=1e-3, istat=5): plane
+                          k=8 and per-field sums of the assembled state, all
+                          tauWall rows, the domInfo.txt text, and sha256 and
+                          first bytes of the Mesh.dat of debugOp=1
+
   post_g1_24x20x16_s50.npz
                           the reference's post-processing chain (oracle/
                           build_post.sh: avg_v1.7, fluc, qInvariant) on the
@@ -30,6 +38,7 @@ Fixtures (all inputs are regenerated from the seed by the tests):
 
     python tests/golden/make_golden.py studies [NAME ...]   # only the study fixtures
     python tests/golden/make_golden.py post       # only the post-processing one
+    python tests/golden/make_golden.py restart    # only the 4-rank restart one
 """
 from __future__ import annotations
 
@@ -107,7 +116,62 @@ def post():
     print("post-processing fixture written", flush=True)
 
 
+RESTART_GRID = (24, 20, 16)
+RESTART_TOPO = (2, 1, 2)
+
+
+def restart():
+    """The reference on 4 MPI ranks (topology 2 x 1 x 2) restarting from its own
+    per-rank files: 20 steps from the seeded start (resOp=1, debugOp=1, so
+    Mesh.dat is written too), then a second build with maxstep 40 restarts
+    (resOp=2) from the 3Dnnn.dat files of step 20.  Both builds keep the
+    reference's E22.15 solution files (build_ref.sh REF_FMT=E22.15), which is
+    what its restart reader expects."""
+    import shutil
+    import subprocess
+    import tempfile
+    n1, n2, n3 = RESTART_GRID
+    v = ic(n1, n2, n3)
+    env = dict(os.environ, REF_FMT="E22.15", OMP_NUM_THREADS="1")
+    exes = []
+    for maxstep in (20, 40):
+        out = subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh"), str(n1), str(n2),
+                              str(n3), str(maxstep), "10"], check=True, capture_output=True,
+                             text=True, env=env).stdout.strip().splitlines()[-1]
+        exes.append(out)
+    root = tempfile.mkdtemp(prefix="ccrst_")
+    nproc = RESTART_TOPO[0] * RESTART_TOPO[1] * RESTART_TOPO[2]
+    prm = formats.Params(dt=0.001, istat=5, topo=RESTART_TOPO, debugOp=1)
+    formats.write_run_dir(root, prm, v)
+    codes, out = os.path.join(root, "codes"), os.path.join(root, "output")
+    mpi = "/opt/conda/bin/mpiexec"
+    subprocess.run([mpi, "-n", str(nproc), exes[0]], cwd=codes, check=True, capture_output=True,
+                   text=True, env=env, timeout=600)
+    dom20 = open(os.path.join(out, "domInfo.txt")).read()
+    mesh = open(os.path.join(codes, "Mesh.dat"), "rb").read()
+    formats.write_param(os.path.join(root, "param.dat"),
+                        formats.Params(dt=0.001, istat=5, topo=RESTART_TOPO, resOp=2))
+    p = subprocess.run([mpi, "-n", str(nproc), exes[1]], cwd=codes, check=True,
+                       capture_output=True, text=True, env=env, timeout=600)
+    t, ns, _, _, _, var = formats.read_rank_files(out)
+    assert ns == 40, (ns, p.stdout[-500:])
+    tau = np.array([np.array(l.split(), dtype=np.float64)
+                    for l in open(os.path.join(out, "tauWall.dat")) if l.strip()])
+    np.savez(
+        os.path.join(HERE, f"restart_{n1}x{n2}x{n3}_topo212_s20_s40.npz"),
+        plane_k8=var[:, 7], sums=var.reshape(7, -1).sum(axis=1), tauWall=tau,
+        dominfo=np.frombuffer(dom20.encode(), dtype=np.uint8),
+        mesh_sha256=np.frombuffer(hashlib.sha256(mesh).digest(), dtype=np.uint8),
+        mesh_head=np.frombuffer(mesh[:400], dtype=np.uint8),
+    )
+    shutil.rmtree(root, ignore_errors=True)
+    print("restart fixture written", flush=True)
+
+
 def main():
+    if sys.argv[1:] == ["restart"]:
+        restart()
+        return
     if sys.argv[1:2] == ["studies"]:
         studies(tuple(sys.argv[2:]) or STUDIES)
         return
@@ -138,6 +202,7 @@ def main():
     )
     studies()
     post()
+    restart()
     print("fixtures written to", HERE)
 
 

@@ -126,3 +126,39 @@ def test_driver_study_matches_reference(study):
         assert np.abs(mine - g["tauWall"]).max() <= 1.01e-7
     finally:
         shutil.rmtree(root)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fastio", ["1", "0"])
+def test_driver_restart_matches_the_reference_restart(fastio):
+    """Topology 2 x 1 x 2 in param.dat: 20 steps from the seeded start, then a
+    restart (resOp = 2) from the per-rank files of step 20 to step 40, against
+    the reference doing the same on 4 MPI ranks (tests/golden/restart_*.npz).
+    The restart zeroes the forcing until the next statistics step in both
+    (main_v1.9.f90:11,33), which the tauWall rows after step 20 show."""
+    print(f"\n[case] driver restart 24x20x16 topology 2x1x2, fastio {fastio}", flush=True)
+    g = np.load(os.path.join(GOLD, "restart_24x20x16_topo212_s20_s40.npz"))
+    root = tempfile.mkdtemp(prefix="ccdrv_")
+    var = formats.round_e22_15(formats.laminar_ic(24, 20, 16))
+    formats.write_run_dir(root, formats.Params(dt=0.001, istat=5, topo=(2, 1, 2)), var)
+    out = os.path.join(root, "output")
+    try:
+        env = {"CC_MAXSTEP": "20", "CC_NWRITE": "10", "CC_FASTIO": fastio}
+        p = run_driver(root, env)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        assert open(os.path.join(out, "domInfo.txt")).read() == bytes(g["dominfo"]).decode()
+        formats.write_param(os.path.join(root, "param.dat"),
+                            formats.Params(dt=0.001, istat=5, topo=(2, 1, 2), resOp=2))
+        p = run_driver(root, dict(env, CC_MAXSTEP="40"))
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        t, ns, x1, x2, x3, v = formats.read_rank_files(out)
+        assert ns == 40 and abs(t - 0.04) < 1e-9
+        for c in range(7):
+            ref = g["plane_k8"][c]
+            assert np.linalg.norm(v[c, 7] - ref) <= 1e-12 * np.linalg.norm(ref), c
+        assert np.allclose(v.reshape(7, -1).sum(axis=1), g["sums"], rtol=1e-12, atol=1e-9)
+        mine = rows(os.path.join(out, "tauWall.dat"))
+        assert mine.shape == g["tauWall"].shape
+        assert np.abs(mine - g["tauWall"]).max() <= 1.01e-7
+    finally:
+        shutil.rmtree(root)
