@@ -1,13 +1,21 @@
 """Learned subgrid closure: training data from the GPU DNS, training with the
 library's own BF16-MFMA kernels, and the coarse run it drives.
 
-    python -m compchan3dmpi_amd.closure [--quick]
+    python -m compchan3dmpi_amd.closure [--quick] [--fixture PATH]
     python -m compchan3dmpi_amd.closure --bench-mlp N   # training throughput
 
 prints one JSON line comparing the coarse run (nP = [64,32,48], the
 64x48x32 grid) with and without the closure against the DNS it was trained
 on (nP = [256,128,192] unless --quick): time-averaged wall shear (skin
 friction), mean-velocity profile, and GPU time per unit of simulated time.
+
+The DNS is turbulent.  The seeded 5 % start of the other configurations
+stays laminar at Re 3000 (its noise decays), and a laminar flow has no
+subgrid stress to learn, so the DNS here starts from a large-scale seeded
+perturbation (turbulent_start) that goes turbulent on 128x64x96, is
+interpolated onto 256x128x192, develops its small scales there in the mixed
+mode, and is then run in FP64 for the window that judges the coarse runs and
+provides the training snapshots.
 """
 from __future__ import annotations
 
@@ -94,36 +102,44 @@ def to_coarse(var, coarse, Mac=1.5):
     return v
 
 
-def run_dns(nP, dt, warm, nsnap, every, coarse, precision="mixed", istat=10, log=print):
-    """DNS from the seeded start: warm-up, then nsnap filtered snapshots.
-    Returns training samples, the wall shear and mean profile over the
-    window, the coarse initial state (the warmed-up DNS on the coarse
-    grid) and the GPU seconds per unit of simulated time."""
+def turbulent_start(nP, amp=0.3, seed=7, modes=4, Mac=1.5):
+    """Laminar Poiseuille flow plus a large-scale seeded perturbation of the
+    three velocity components: per component a sum of Fourier modes with
+    |kx|, |ky| <= modes (random normal amplitudes, uniform phases; numpy's
+    PCG64 with this seed) normalised to unit rms over the plane, times a
+    random wall-normal shape of the first three sine modes bounded by
+    amp x3 (2 - x3) / max, so that it vanishes at the walls.  E is made
+    consistent with T = 1; the periodic duplicates are refilled."""
     n1, n2, n3 = nP
-    ch = Channel(n1, n2, n3, dt=dt, istat=istat, precision=precision)
-    ch.upload(formats.laminar_ic(n1, n2, n3))
-    t0 = time.perf_counter()
-    if ch.step(warm):
-        raise RuntimeError("DNS blew up during warm-up")
-    t_warm = time.perf_counter() - t0
-    v0 = to_coarse(ch.download()[0], coarse)
-    Xs, Ts, taus, profs = [], [], [], []
-    t_win = 0.0
-    for s in range(nsnap):
-        t1 = time.perf_counter()
-        if ch.step(every):
-            raise RuntimeError("DNS blew up")
-        t_win += time.perf_counter() - t1
-        X, T = ch.filter_samples()
-        Xs.append(X)
-        Ts.append(T)
-        taus.append(ch.stats()[0].copy())
-        profs.append(mean_profile(ch.download()[0]))
-        log(f"  DNS snapshot {s + 1}/{nsnap}: {X.shape[0]} samples, tau = {taus[-1]}")
-    cost = (t_warm + t_win) / ((warm + nsnap * every) * dt)
-    ch.close()
-    return (np.concatenate(Xs), np.concatenate(Ts), np.array(taus), np.mean(profs, axis=0),
-            v0, cost)
+    v = formats.laminar_ic(n1, n2, n3, Mac=Mac, amp=0.0)
+    x1, x2, x3 = formats.mesh(n1, n2, n3)
+    X, Y = np.meshgrid(x1[1:-1], x2[1:-1], indexing="xy")
+    rng = np.random.default_rng(seed)
+    shape = (x3 * (2 - x3))[1:-1]
+    for c in (1, 2, 3):
+        f = np.zeros((n2 - 2, n1 - 2))
+        for kx in range(0, modes + 1):
+            for ky in range(0, modes + 1):
+                if kx == 0 and ky == 0:
+                    continue
+                a, ph = rng.normal(), rng.uniform(0, 2 * np.pi)
+                f += a * np.cos(2 * np.pi * (kx * X / x1[-1] + ky * Y / x2[-1]) + ph)
+        f /= f.std()
+        g = np.zeros(n3 - 2)
+        for m in range(1, 4):
+            g += rng.normal() * np.sin(m * np.pi * x3[1:-1] / 2)
+        g = g / np.abs(g).max() * shape / shape.max()
+        v[c, 1:-1, 1:-1, 1:-1] += amp * g[:, None, None] * f[None, :, :]
+    ggM = formats.GAMA * (formats.GAMA - 1) * Mac ** 2
+    v[4] = v[5] + ggM * 0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2)
+    v[:, :, :, 0] = v[:, :, :, n1 - 2]
+    v[:, :, :, n1 - 1] = v[:, :, :, 1]
+    v[:, :, 0, :] = v[:, :, n2 - 2, :]
+    v[:, :, n2 - 1, :] = v[:, :, 1, :]
+    return v
+
+
+regrid = to_coarse  # the same interpolation serves any target grid
 
 
 def run_coarse(nP, v0, dt, nsnap, every, mlp=None, norm=None, precision="mixed", istat=10):
@@ -148,7 +164,9 @@ def run_coarse(nP, v0, dt, nsnap, every, mlp=None, norm=None, precision="mixed",
 
 # A coarse run whose mean skin friction is further than this from the DNS's
 # over the same window (or that blows up) is not accepted: the full DNS is
-# the fallback, and its result is reported in its place.
+# the fallback a user runs instead.  The experiment reports every coarse
+# run's own errors with the flag; it never puts the DNS's numbers in their
+# place.
 FALLBACK_CF_TOL = 0.03
 
 
@@ -163,65 +181,167 @@ def accepted(taus, cf_dns, tol=FALLBACK_CF_TOL):
     return bool(abs(0.5 * (tw[0] - tw[1]) - cf_dns) <= tol * abs(cf_dns))
 
 
-def experiment(quick=False, log=print):
-    """DNS (training data, reference and fallback) -> training -> coarse runs
-    started from the filtered DNS, with and without the closure, over the
-    same window of simulated time, each judged against the DNS skin
-    friction (and replaced by the DNS when outside FALLBACK_CF_TOL)."""
-    fine = (64, 32, 48) if quick else (256, 128, 192)
-    coarse = (18, 10, 12) if quick else (64, 32, 48)
-    dt_f = 1.5e-3 if quick else 9e-5
-    dt_c = 2e-3 if quick else 1.5e-3
-    T_warm, T_win = (0.3, 0.3) if quick else (2.7, 2.7)
-    nsnap = 4 if quick else 6
-    warm = int(round(T_warm / dt_f))
+def box_profile(prof_fine, n3_fine):
+    """A fine mean profile averaged over the filter's 4-plane boxes, and the
+    boxes' mean heights."""
+    _, _, x3f = formats.mesh(4, 4, n3_fine)
+    nb = n3_fine // 4
+    zc = np.array([x3f[4 * b:4 * b + 4].mean() for b in range(nb)])
+    pf = np.array([prof_fine[4 * b:4 * b + 4].mean() for b in range(nb)])
+    return zc, pf
+
+
+def judge(taus, profs, cf_dns, zc, pf, coarse):
+    """Errors of a coarse run against the DNS: mean skin friction over the
+    window, and the time-mean streamwise velocity interpolated to the box
+    heights (max error over the profile, relative to its maximum)."""
+    taus = np.asarray(taus)
+    tw = taus.mean(axis=0)
+    cf = 0.5 * (tw[0] - tw[1])
+    _, _, x3c = formats.mesh(*coarse)
+    prof = np.interp(zc, x3c, np.asarray(profs).mean(axis=0))
+    return {"tau_wall": [float(tw[0]), float(tw[1])],
+            "skin_friction": float(cf),
+            "skin_friction_rel_err": float(abs(cf - cf_dns) / abs(cf_dns)),
+            "mean_u_profile_rel_err": float(np.abs(prof - pf).max() / np.abs(pf).max()),
+            "accepted": accepted(taus, cf_dns)}
+
+
+CONFIG5 = dict(medium=(128, 64, 96), fine=(256, 128, 192), coarse=(64, 32, 48),
+               amp=0.3, seed=7, T_trans=60.0, T_dev=10.0, T_win=4.0, nsnap=8,
+               dt_f=9e-5, dt_c=1.5e-3, epochs=20, batch=8192)
+QUICK = dict(medium=None, fine=(64, 32, 48), coarse=(18, 10, 12), amp=0.0, seed=7,
+             T_trans=0.0, T_dev=0.3, T_win=0.3, nsnap=4, dt_f=1.5e-3, dt_c=2e-3, epochs=6,
+             batch=2048)
+
+
+def develop(cfg, log=print):
+    """The DNS state at the start of the judging window (fine grid) and the
+    GPU seconds it took: transition on the medium grid from the large-scale
+    perturbation, interpolation to the fine grid, development there (mixed);
+    the quick configuration develops the seeded laminar start instead."""
+    t0 = time.perf_counter()
+    if cfg["medium"]:
+        v = turbulent_start(cfg["medium"], cfg["amp"], cfg["seed"])
+        dt_m = formats.cfl_dt(v, 1.5)
+        ch = Channel(*cfg["medium"], dt=dt_m, istat=10, precision="mixed")
+        ch.upload(v)
+        n = int(round(cfg["T_trans"] / dt_m))
+        chunk = int(round(5.0 / dt_m))
+        for s0 in range(0, n, chunk):
+            if ch.step(min(chunk, n - s0)):
+                raise RuntimeError("transition run blew up")
+            log(f"  transition {cfg['medium']}: t = {(s0 + min(chunk, n - s0)) * dt_m:.1f}, "
+                f"tau = {ch.stats()[0]}")
+        v = regrid(ch.download()[0], cfg["fine"])
+        ch.close()
+    else:
+        v = formats.laminar_ic(*cfg["fine"])
+    ch = Channel(*cfg["fine"], dt=cfg["dt_f"], istat=10, precision="mixed")
+    ch.upload(v)
+    n = int(round(cfg["T_dev"] / cfg["dt_f"]))
+    chunk = max(1, int(round(2.0 / cfg["dt_f"])))
+    for s0 in range(0, n, chunk):
+        if ch.step(min(chunk, n - s0)):
+            raise RuntimeError("DNS blew up while developing")
+        log(f"  DNS {cfg['fine']} developing: t = {(s0 + min(chunk, n - s0)) * cfg['dt_f']:.2f}, "
+            f"tau = {ch.stats()[0]}")
+    v = ch.download()[0]
+    ch.close()
+    return v, time.perf_counter() - t0
+
+
+def experiment(quick=False, log=print, fixture=None):
+    """Turbulent DNS -> FP64 judging window with filtered snapshots ->
+    training -> coarse runs from the filtered DNS at the window start, with
+    and without the closure, over the same window; each coarse run is judged
+    against the DNS skin friction and mean profile and reported as it is
+    (with whether the DNS fallback would have been taken).  fixture: path of
+    an .npz receiving the coarse start, the trained network and the DNS
+    figures, for a test to replay the two coarse runs."""
+    cfg = QUICK if quick else CONFIG5
+    fine, coarse, dt_f, dt_c = cfg["fine"], cfg["coarse"], cfg["dt_f"], cfg["dt_c"]
+    vw, t_dev = develop(cfg, log)
+    v0 = to_coarse(vw, coarse)
+    # the judging window in FP64: training snapshots, wall shear, mean profile
+    nsnap, T_win = cfg["nsnap"], cfg["T_win"]
     every = int(round(T_win / nsnap / dt_f))
-    log(f"DNS {fine}: {warm} warm-up steps, {nsnap} snapshots every {every} steps")
-    X, T, tau_dns, prof_dns, v0, cost_dns = run_dns(fine, dt_f, warm, nsnap, every, coarse, log=log)
+    ch = Channel(*fine, dt=dt_f, istat=10, precision="fp64")
+    ch.upload(vw)
+    Xs, Ts, taus, profs = [], [], [], []
+    t_win = 0.0
+    for s_ in range(nsnap):
+        t1 = time.perf_counter()
+        if ch.step(every):
+            raise RuntimeError("FP64 DNS blew up")
+        t_win += time.perf_counter() - t1
+        X, T = ch.filter_samples()
+        Xs.append(X)
+        Ts.append(T)
+        taus.append(ch.stats()[0].copy())
+        profs.append(mean_profile(ch.download()[0]))
+        log(f"  FP64 DNS snapshot {s_ + 1}/{nsnap}: {X.shape[0]} samples, tau = {taus[-1]}")
+    ch.close()
+    cost_dns = t_win / (nsnap * every * dt_f)
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    tau_dns, prof_dns = np.array(taus), np.mean(profs, axis=0)
     Xn, Tn, norm = normalise(X, T)
     mlp = Mlp(seed=1)
-    hist = mlp.train(Xn, Tn, epochs=6 if quick else 20, batch=2048 if quick else 8192,
-                     lr=2e-3, seed=2)
+    hist = mlp.train(Xn, Tn, epochs=cfg["epochs"], batch=cfg["batch"], lr=2e-3, seed=2)
     log(f"training loss per epoch: {np.array2string(hist, precision=4)}")
     ce = int(round(T_win / nsnap / dt_c))
     td = tau_dns.mean(axis=0)
     cf_dns = 0.5 * (td[0] - td[1])
-    res = {}
+    zc, pf = box_profile(prof_dns, fine[2])
+    out = {"dns_grid": list(fine), "coarse_grid": list(coarse), "samples": int(X.shape[0]),
+           "config": {k: (list(v) if isinstance(v, tuple) else v) for k, v in cfg.items()},
+           "dns_window": "FP64", "dns_develop_gpu_s": t_dev,
+           "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+           "tau_wall_dns": [float(td[0]), float(td[1])], "skin_friction_dns": float(cf_dns)}
     for name, (m, nv) in (("no_closure", (None, None)), ("closure", (mlp, norm))):
         try:
-            taus, profs, cost = run_coarse(coarse, v0, dt_c, nsnap, ce, m, nv)
-            tw = taus.mean(axis=0)
-            ok = accepted(taus, cf_dns)
-        except RuntimeError as e:  # the coarse run blew up
+            ct, cp, cost = run_coarse(coarse, v0, dt_c, nsnap, ce, m, nv)
+        except RuntimeError as e:  # the coarse run blew up: not accepted
             log(f"{name}: {e}")
-            ok = False
-        if not ok:
-            # the full DNS stands in for a coarse run it does not accept
-            log(f"{name}: outside {FALLBACK_CF_TOL:.0%} of the DNS skin friction, DNS result used")
-            res[name] = {"tau_wall": td, "cost": cost_dns, "profile": prof_dns,
-                         "fell_back_to_dns": True}
+            out[name] = {"blew_up": True, "accepted": False}
             continue
-        res[name] = {"tau_wall": tw, "cost": cost, "profile": profs.mean(axis=0),
-                     "fell_back_to_dns": False}
-    _, _, x3f = formats.mesh(*fine)
-    zc = np.array([x3f[4 * b:4 * b + 4].mean() for b in range(fine[2] // 4)])
-    _, _, x3c = formats.mesh(*coarse)
-    pf = np.array([prof_dns[4 * b:4 * b + 4].mean() for b in range(fine[2] // 4)])
-    out = {"dns_grid": list(fine), "coarse_grid": list(coarse), "samples": int(X.shape[0]),
-           "window": [T_warm, T_warm + T_win],
-           "train_loss_first_last": [float(hist[0]), float(hist[-1])],
-           "tau_wall_dns": [float(td[0]), float(td[1])]}
-    for name, r in res.items():
-        tw = r["tau_wall"]
-        cf = 0.5 * (tw[0] - tw[1])
-        prof = pf if r["fell_back_to_dns"] else np.interp(zc, x3c, r["profile"])
-        out[name] = {"tau_wall": [float(tw[0]), float(tw[1])],
-                     "skin_friction_rel_err": float(abs(cf - cf_dns) / abs(cf_dns)),
-                     "mean_u_profile_rel_err": float(np.abs(prof - pf).max() / np.abs(pf).max()),
-                     "cost_ratio_vs_dns": float(cost_dns / r["cost"]),
-                     "fell_back_to_dns": r["fell_back_to_dns"]}
+        r = judge(ct, cp, cf_dns, zc, pf, coarse)
+        r["cost_ratio_vs_dns"] = float(cost_dns / cost)
+        r["blew_up"] = False
+        out[name] = r
+        log(f"{name}: skin friction {r['skin_friction']:.5e} vs DNS {cf_dns:.5e} "
+            f"({100 * r['skin_friction_rel_err']:.2f} %), profile {r['mean_u_profile_rel_err']:.3e}, "
+            f"accepted {r['accepted']}")
+    if fixture:
+        core = (slice(None), slice(None), slice(1, -1), slice(1, -1))
+        prim = np.stack([v0[0], v0[1], v0[2], v0[3], v0[5]])
+        np.savez_compressed(
+            fixture, coarse=np.array(coarse), prim=prim[core].astype(np.float32),
+            params=mlp.params.astype(np.float32), norm=norm, dt=dt_c, steps=nsnap * ce,
+            every=ce, tau_dns=tau_dns, cf_dns=cf_dns, box_z=zc, box_u=pf,
+            fine=np.array(fine))
     mlp.close()
     return out
+
+
+def coarse_start(prim, coarse, Mac=1.5):
+    """A coarse state from the unique-node primitives rho, u, v, w, T (the
+    closure fixture's float32 copy): E and p recomputed, the walls' velocity
+    zero, the periodic duplicates refilled."""
+    c1, c2, c3 = coarse
+    v = np.zeros((7, c3, c2, c1))
+    for f, c in enumerate((0, 1, 2, 3, 5)):
+        v[c, :, 1:-1, 1:-1] = prim[f]
+    v[1:4, 0] = 0.0
+    v[1:4, -1] = 0.0
+    ggM = formats.GAMA * (formats.GAMA - 1) * Mac ** 2
+    v[4] = v[5] + ggM * 0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2)
+    v[6] = v[0] * v[5] / (formats.GAMA * Mac ** 2)
+    v[:, :, :, 0] = v[:, :, :, c1 - 2]
+    v[:, :, :, c1 - 1] = v[:, :, :, 1]
+    v[:, :, 0, :] = v[:, :, c2 - 2, :]
+    v[:, :, c2 - 1, :] = v[:, :, 1, :]
+    return v
 
 
 # Multiply-adds per sample of the BF16 MFMA layers as the kernels run them
@@ -295,6 +415,8 @@ def bench_coarse(steps=500, nP=(64, 32, 48), dt=4e-4, precision="mixed"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--fixture", default=None,
+                    help="write the coarse start, trained network and DNS figures to this .npz")
     ap.add_argument("--bench-mlp", type=int, default=0, metavar="N",
                     help="time training on N device-resident samples instead")
     ap.add_argument("--bench-coarse", type=int, default=0, metavar="STEPS",
@@ -306,7 +428,8 @@ def main():
     if a.bench_coarse:
         print(json.dumps(bench_coarse(a.bench_coarse)), flush=True)
         return
-    out = experiment(a.quick, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    out = experiment(a.quick, log=lambda *x: print(*x, file=sys.stderr, flush=True),
+                     fixture=a.fixture)
     print(json.dumps(out), flush=True)
 
 
