@@ -77,19 +77,26 @@ __device__ __forceinline__ float rcp<float>(float x) {
 }
 
 // T**0.7 for T > 0 (mu = kt = T**0.7, main_v1.9.f90:643,720).
-// Double: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+// Double, T in [2^-100, 2^100]: the hardware FP32 log2/exp2 give a seed y0
+// within a few FP32 ulp; one Halley step on f(y) = y^10 - T^7 (0.7 = 7/10),
+//   y1 = y0 (9 y0^10 + 11 T^7) / (11 y0^10 + 9 T^7),
+// converges cubically (relative error ~8 e0^3, below 1e-19), leaving the
+// rounding of the powers and the quotient: within 5 ulp of the correctly
+// rounded T**0.7 (2 ulp for 96 % of arguments), which the 1e-12 FP64
+// tolerance absorbs.  It costs ~20 instructions where a log/exp series in
+// FP64 costs ~80.  Outside that range the series below (2 ulp) serves all
+// positive finite T; 0 and +inf return themselves (0**0.7 = 0,
+// inf**0.7 = inf), negative T and NaN give NaN, as the reference's
+// T**0.7d0, so that a negative temperature propagates NaN.
+// Series: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
 // s = (m-1)/(m+1), |s| < 0.172, by its odd series to s^21; then
 // 2^(0.7 log2 x) = 2^n exp(f ln 2) with n = rint(.), |f| <= 1/2 and the
-// exponential by its Taylor series to degree 14.  Error <= 2 ulp over the
-// range a temperature can take, which the 1e-12 FP64 tolerance absorbs.
-// Outside x > 0 the series do not apply; the result is then what pow(x, 0.7)
-// gives there: 0 at x = 0, +inf at +inf, NaN for x < 0 and NaN, so that a
-// negative temperature propagates NaN exactly as the reference's T**0.7d0.
-// Float: the hardware log2/exp2 (v_log_f32, v_exp_f32), which give those
+// exponential by its Taylor series to degree 14.
+// Float: the hardware log2/exp2 (v_log_f32, v_exp_f32), which give the
 // special values themselves.  The library's log2f/exp2f wrap the same
 // instructions in rescaling for denormal inputs and results (12 more
 // instructions), which a temperature never reaches.
-__device__ __forceinline__ double pow07d_series(double x) {
+__device__ __noinline__ double pow07d_series(double x) {
   int e;
   double m = frexp(x, &e);
   if (m < 0.70710678118654752440) {
@@ -128,14 +135,17 @@ __device__ __forceinline__ double pow07d_series(double x) {
   q = fma(q, g, 0.5);
   q = fma(q, g, 1.0);
   q = fma(q, g, 1.0);
-  return ldexp(q, (int)n);
-}
-__device__ __forceinline__ double pow07d(double x) {
-  const double r = pow07d_series(x);
-  // finite positive x: the series; x = 0 and x = +inf return x itself
-  // (0**0.7 = 0, inf**0.7 = inf); x < 0 and NaN: NaN
+  const double r = ldexp(q, (int)n);
   if (x > 0.0 && x < __builtin_huge_val()) return r;
   return (x == 0.0 || x > 0.0) ? x : __builtin_nan("");
+}
+__device__ __forceinline__ double pow07d(double x) {
+  if (!(x >= 0x1p-100 && x <= 0x1p100)) return pow07d_series(x);  // NaN takes this branch
+  const double y = (double)__builtin_amdgcn_exp2f(0.7f * __builtin_amdgcn_logf((float)x));
+  const double x2 = x * x, c = (x2 * x2) * (x2 * x);  // x^7
+  const double y2 = y * y, y4 = y2 * y2, y10 = (y4 * y4) * y2;
+  const double num = fma(9.0, y10, 11.0 * c), den = fma(11.0, y10, 9.0 * c);
+  return y * num * rcp<double>(den);
 }
 
 template <typename T>
@@ -297,6 +307,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     }
   };
 
+  T u_t = T(0), v_t = T(0), mu_t = T(0);  // u, v, mu of the own node of plane t (phase 1 -> 2)
   const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
   const int tlast = min(n3 - 1, tend);             // last plane loaded
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
@@ -322,9 +333,22 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     }
   };
   if constexpr (kAhead) load_plane(tb);
+  // the forcing's wall-shear difference is fixed for the launch
+  const double tdiff = cread(a.force, 0) - cread(a.force, 1);
   for (int t = tb; t <= tend; t++) {
     const int s = t & 1;
     const bool have_plane = t <= n3 - 1;
+    // This iteration's metric and forcing values, loaded here and made
+    // resident before the phase-1 barrier (the empty asm uses them there):
+    // a scalar load still in flight would turn every later LDS wait into a
+    // full drain (scalar loads complete out of order under the same
+    // counter), and phases 2-4 are where the LDS waits are.  Indices past
+    // the grid are clamped; those values go unused.
+    const double jac_t = cread(a.jac, min(t, n3 - 1));      // plane t
+    const double iJ_t = cread(a.invJac, min(t, n3 - 1));    // plane p+1 = t
+    const double iJ_p = cread(a.invJac, max(t - 1, 0));     // plane p = t-1
+    const double iJ_q = cread(a.invJac, max(t - 2, 0));     // plane q = t-2
+    const double rb_q = cread(a.force, 2 + max(t - 2, 0));  // rho_bar(q)
     // ---- phase 1: load plane t ----------------------------------------
     // plane t-2 of the own column leaves the LDS slot this iteration
 #pragma unroll
@@ -341,6 +365,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         for (int c = 0; c < 5; c++) Uo[c] = Uin[c * F + n];
       }
       prim(o_own, t, u, v, w, Tt, p, mu, Uo);
+      u_t = u;
+      v_t = v;
+      mu_t = mu;
       sl[s][0][r_own] = u;
       sl[s][1][r_own] = v;
       sl[s][2][r_own] = w;
@@ -360,7 +387,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
 #pragma unroll
       for (int c = 0; c < 5; c++) Uw[4][c] = T(Uo[c]);
       // z scheme of this plane (main_v1.9.f90:409-416)
-      const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(cread(a.jac, t)) > T(ph.Pez) * mu;
+      const bool qz = T(ph.Re) * fabs(T(Uo[3])) * T(ph.dXi) * T(jac_t) > T(ph.Pez) * mu;
       zq = (zq << 1) | (qz ? 1u : 0u);
       if (has_ring) {
         if constexpr (kAhead) {
@@ -405,6 +432,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       Sg[1][c] = Sg[2][c];
       Sg[2][c] = T(0);
     }
+    asm volatile("" ::"s"(jac_t), "s"(iJ_t), "s"(iJ_p), "s"(iJ_q), "s"(rb_q));
     __syncthreads();
     {
       const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -432,7 +460,83 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     }
 
     // ---- phase 2: x, y convective fluxes of plane t (main_v1.9.f90:333-407)
-    if (have_plane && t >= kb && t < ke) {
+    // Each direction first gathers its whole stencil from LDS, the node's own
+    // state coming from the z window and phase 1's registers, and only then
+    // computes: the reads go out back to back and their latencies overlap,
+    // instead of one LDS round trip per stencil point (sched_barrier keeps
+    // the compiler from sinking the reads back to their uses).  FP64 only:
+    // the FP32 kernels, at three waves per SIMD, hide the latency with their
+    // third wave and spill under the gathered registers.
+    if (have_plane && t >= kb && t < ke && sizeof(T) == 8) {
+      const T* U5 = sub;
+      const int ro = r_own;
+      // x
+      {
+        const T* ul = sl[s][0];
+        if (T(ph.Re) * fabs(Uw[4][1]) * T(ph.dx) > T(ph.Pex) * mu_t) {
+          T qx[5][4];
+          const T um1 = ul[ro - 1], up1 = ul[ro + 1];
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const T* q = U5 + c * NR + ro;
+            qx[c][0] = q[-2];
+            qx[c][1] = q[-1];
+            qx[c][2] = q[1];
+            qx[c][3] = q[2];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          const T t1 = (u_t + up1) * half, t2 = (u_t + um1) * half;
+#pragma unroll
+          for (int c = 0; c < 5; c++)
+            Sg[2][c] += quickT<T>(t1, t2, qx[c][0], qx[c][1], Uw[4][c], qx[c][2], qx[c][3]) *
+                        (invdx * T(0.125));
+        } else {
+          T qx[5][4];
+          const T um2 = ul[ro - 2], um1 = ul[ro - 1], up1 = ul[ro + 1], up2 = ul[ro + 2];
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const T* q = U5 + c * NR + ro;
+            qx[c][0] = q[-2];
+            qx[c][1] = q[-1];
+            qx[c][2] = q[1];
+            qx[c][3] = q[2];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int c = 0; c < 5; c++)
+            Sg[2][c] += (-up2 * qx[c][3] + T(8) * up1 * qx[c][2] - T(8) * um1 * qx[c][1] +
+                         um2 * qx[c][0]) * (invdx * T(kOne12th));
+        }
+      }
+      // y: the Peclet test differs from node to node within most waves, so
+      // both schemes are evaluated from one read of the stencil and the
+      // node's own is selected (a divergent branch would read it twice)
+      {
+        const T* vl = sl[s][1];
+        const bool quick = T(ph.Re) * fabs(Uw[4][2]) * T(ph.dy) > T(ph.Pey) * mu_t;
+        T qy[5][4];
+        const T vm2 = vl[ro - 2 * RX], vm1 = vl[ro - RX], vp1 = vl[ro + RX], vp2 = vl[ro + 2 * RX];
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+          const T* q = U5 + c * NR + ro;
+          qy[c][0] = q[-2 * RX];
+          qy[c][1] = q[-RX];
+          qy[c][2] = q[RX];
+          qy[c][3] = q[2 * RX];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const T t1 = (v_t + vp1) * half, t2 = (v_t + vm1) * half;
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+          const T fq = quickT<T>(t1, t2, qy[c][0], qy[c][1], Uw[4][c], qy[c][2], qy[c][3]) *
+                       (invdy * T(0.125));
+          const T fc = (-vp2 * qy[c][3] + T(8) * vp1 * qy[c][2] - T(8) * vm1 * qy[c][1] +
+                        vm2 * qy[c][0]) * (invdy * T(kOne12th));
+          Sg[2][c] += quick ? fq : fc;
+        }
+      }
+    }
+    if (have_plane && t >= kb && t < ke && sizeof(T) == 4) {
       const T* U5 = sub;
       const int ro = r_own;
       const T mu = spm[s][1][ro];
@@ -456,9 +560,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
           }
         }
       }
-      // y: the Peclet test differs from node to node within most waves, so
-      // both schemes are evaluated from one read of the stencil and the
-      // node's own is selected (a divergent branch would read it twice)
+      // y: both schemes from one read of the stencil, the node's own selected
       {
         const T rv = U5[2 * NR + ro];
         const T* vl = sl[s][1];
@@ -489,37 +591,57 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     auto zder = [&](T qm, T q0, T qp) -> T {
       if (p == 0) return (qp - q0) * T(ph.invdz1);
       if (p == n3 - 1) return (q0 - qm) * T(ph.invdzN);
-      return (qp - qm) * (half * invdXi * T(cread(a.invJac, p)));
+      return (qp - qm) * (half * invdXi * T(iJ_p));
     };
-    // derivatives at a region node of plane p; z values supplied
-    auto derivs = [&](int r, const T zm[3], const T zp[3], T zTm, T zTp, Deriv<T>& D) {
-      const T* u = sl[sp][0];
-      const T* v = sl[sp][1];
-      const T* w = sl[sp][2];
-      const T* Tt = sl[sp][3];
-      D.dudx = (u[r + 1] - u[r - 1]) * hx;
-      D.dvdx = (v[r + 1] - v[r - 1]) * hx;
-      D.dwdx = (w[r + 1] - w[r - 1]) * hx;
-      D.dTdx = (Tt[r + 1] - Tt[r - 1]) * hx;
-      D.dudy = (u[r + RX] - u[r - RX]) * hy;
-      D.dvdy = (v[r + RX] - v[r - RX]) * hy;
-      D.dwdy = (w[r + RX] - w[r - RX]) * hy;
-      D.dTdy = (Tt[r + RX] - Tt[r - RX]) * hy;
-      D.dudz = zder(zm[0], u[r], zp[0]);
-      D.dvdz = zder(zm[1], v[r], zp[1]);
-      D.dwdz = zder(zm[2], w[r], zp[2]);
-      D.dTdz = zder(zTm, Tt[r], zTp);
+    // derivatives at a region node of plane p; z values supplied.  The
+    // node's whole x/y stencil (u, v, w, T at r, r+-1, r+-RX) and its p, mu
+    // are read before anything is computed, so that the 22 LDS reads overlap
+    // (see phase 2).
+    struct Sten {
+      T nb[4][4];  // u, v, w, T at r-1, r+1, r-RX, r+RX
+      T c[4];      // u, v, w, T at r
+      T pr, mu;
+    };
+    auto gather = [&](int r, Sten& G) {
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        const T* q = sl[sp][f];
+        G.nb[f][0] = q[r - 1];
+        G.nb[f][1] = q[r + 1];
+        G.nb[f][2] = q[r - RX];
+        G.nb[f][3] = q[r + RX];
+        G.c[f] = q[r];
+      }
+      G.pr = spm[sp][0][r];
+      G.mu = spm[sp][1][r];
+    };
+    auto derivs_of = [&](const Sten& G, const T zm[3], const T zp[3], T zTm, T zTp, Deriv<T>& D) {
+      D.dudx = (G.nb[0][1] - G.nb[0][0]) * hx;
+      D.dvdx = (G.nb[1][1] - G.nb[1][0]) * hx;
+      D.dwdx = (G.nb[2][1] - G.nb[2][0]) * hx;
+      D.dTdx = (G.nb[3][1] - G.nb[3][0]) * hx;
+      D.dudy = (G.nb[0][3] - G.nb[0][2]) * hy;
+      D.dvdy = (G.nb[1][3] - G.nb[1][2]) * hy;
+      D.dwdy = (G.nb[2][3] - G.nb[2][2]) * hy;
+      D.dTdy = (G.nb[3][3] - G.nb[3][2]) * hy;
+      D.dudz = zder(zm[0], G.c[0], zp[0]);
+      D.dvdz = zder(zm[1], G.c[1], zp[1]);
+      D.dwdz = zder(zm[2], G.c[2], zp[2]);
+      D.dTdz = zder(zTm, G.c[3], zTp);
     };
     T Fx[4] = {T(0), T(0), T(0), T(0)}, Gy[4] = {T(0), T(0), T(0), T(0)};
     if (need_h) {
       // own node: all three point fluxes; hD stays in registers
       const int r = r_own;
       const T zm[3] = {zU[0], zU[1], zU[2]};
-      const T zp[3] = {sl[s][0][r], sl[s][1][r], sl[s][2][r]};
       Deriv<T> D;
-      derivs(r, zm, zp, zU[3], sl[s][3][r], D);
-      const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
-      const T pr = spm[sp][0][r], mu = spm[sp][1][r];
+      Sten G;
+      gather(r, G);
+      const T zp[3] = {sl[s][0][r], sl[s][1][r], sl[s][2][r]};
+      const T zTp = sl[s][3][r];
+      __builtin_amdgcn_sched_barrier(0);
+      derivs_of(G, zm, zp, zU[3], zTp, D);
+      const T u = G.c[0], v = G.c[1], w = G.c[2], pr = G.pr, mu = G.mu;
       const T divg = D.dudx + D.dvdy + D.dwdz;
       const T mR = mu * invRe;
       const T sxz = D.dwdx + D.dudz, syz = D.dwdy + D.dvdz, sxy = D.dvdx + D.dudy;
@@ -533,12 +655,12 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       // iteration's phase 4: Sg[0]) and of plane p+1 (begun in this
       // iteration: Sg[2]); no plane of hD needs to be kept
       if (p - 1 >= kb && p - 1 < ke) {
-        const T cm = half * invdXi * T(cread(a.invJac, p - 1));
+        const T cm = half * invdXi * T(iJ_q);
 #pragma unroll
         for (int c = 0; c < 4; c++) Sg[0][c + 1] += h[c] * cm;
       }
       if (p + 1 >= kb && p + 1 < ke) {
-        const T cp = half * invdXi * T(cread(a.invJac, p + 1));
+        const T cp = half * invdXi * T(iJ_t);
 #pragma unroll
         for (int c = 0; c < 4; c++) Sg[2][c + 1] -= h[c] * cp;
       }
@@ -576,9 +698,11 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const T zm[3] = {srz[tm][0][zr], srz[tm][1][zr], srz[tm][2][zr]};
         const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
         Deriv<T> D;
-        derivs(r, zm, zp, T(0), T(0), D);
-        const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
-        const T pr = spm[sp][0][r], mu = spm[sp][1][r];
+        Sten G;
+        gather(r, G);
+        __builtin_amdgcn_sched_barrier(0);
+        derivs_of(G, zm, zp, T(0), T(0), D);
+        const T u = G.c[0], v = G.c[1], w = G.c[2], pr = G.pr, mu = G.mu;
         const T divg = D.dudx + D.dvdy + D.dwdz;
         const T mR = mu * invRe;
         const T sxz = D.dwdx + D.dudz, sxy = D.dvdx + D.dudy;
@@ -608,9 +732,11 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const T zm[3] = {srz[tm][0][zr], srz[tm][1][zr], srz[tm][2][zr]};
         const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
         Deriv<T> D;
-        derivs(r, zm, zp, T(0), T(0), D);
-        const T u = sl[sp][0][r], v = sl[sp][1][r], w = sl[sp][2][r];
-        const T pr = spm[sp][0][r], mu = spm[sp][1][r];
+        Sten G;
+        gather(r, G);
+        __builtin_amdgcn_sched_barrier(0);
+        derivs_of(G, zm, zp, T(0), T(0), D);
+        const T u = G.c[0], v = G.c[1], w = G.c[2], pr = G.pr, mu = G.mu;
         const T divg = D.dudx + D.dvdy + D.dwdz;
         const T mR = mu * invRe;
         const T syz = D.dwdy + D.dvdz, sxy = D.dvdx + D.dudy;
@@ -631,16 +757,25 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       const T* gy = sfg + 4 * NFX;
       const int lf = ty * (TX + 2) + (tx + 1);
       const int lg = (ty + 1) * TX + tx;
+      // all 16 reads first, one wait (see phase 2)
+      T d[4][4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        d[c][0] = fx[c * NFX + lf - 1];
+        d[c][1] = fx[c * NFX + lf + 1];
+        d[c][2] = gy[c * NGY + lg - TX];
+        d[c][3] = gy[c * NGY + lg + TX];
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int c = 0; c < 4; c++)
-        Sg[1][c + 1] += (fx[c * NFX + lf + 1] - fx[c * NFX + lf - 1]) * hx +
-                        (gy[c * NGY + lg + TX] - gy[c * NGY + lg - TX]) * hy;
+        Sg[1][c + 1] += (d[c][1] - d[c][0]) * hx + (d[c][3] - d[c][2]) * hy;
     }
 
     // ---- phase 4: complete plane q = t-2 ------------------------------
     const int q = t - 2;
     if (q >= kb && q < ke) {
-      const T iJ = T(cread(a.invJac, q));
+      const T iJ = T(iJ_q);
       // z convective (main_v1.9.f90:409-450); column values at q-2..q+2 are
       // window slots 0..4 (slot 2 = q)
       T H[5];
@@ -662,8 +797,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       }
       // (the z divergence of hD is already in Sg[0])
       double R[5];
-      const double tdiff = cread(a.force, 0) - cread(a.force, 1);
-      const double rb = cread(a.force, 2 + q);
+      const double rb = rb_q;
       const double src1 = 0.5 * tdiff * rb;
       // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
       const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
