@@ -159,6 +159,38 @@ __device__ __forceinline__ float pow07<float>(float x) {
   return __builtin_amdgcn_exp2f(0.7f * __builtin_amdgcn_logf(x));
 }
 
+// Element of a plane array: a wave-uniform base (the field and plane) plus
+// a 32-bit byte offset (the node within the plane).  The access then takes
+// its base from SGPRs and its offset from one VGPR (the saddr form), instead
+// of a 64-bit address per access built by 64-bit vector adds.
+// With kOpaque the base is held opaque in SGPRs as a global-address-space
+// pointer, so that the compiler neither rebuilds the five fields' addresses
+// from the first one with vector adds nor falls back to flat addressing.
+// The FP64 kernels, short of SGPRs for ten such bases (they spill them to
+// VGPR lanes), index plainly by the element offset o.
+template <typename V>
+using gptr = __attribute__((address_space(1))) V*;
+template <bool kOpaque, typename V>
+__device__ __forceinline__ gptr<const V> at_b(const V* base, int o, unsigned boff) {
+  if constexpr (kOpaque) {
+    auto b = (gptr<const char>)base;
+    asm volatile("" : "+s"(b));
+    return (gptr<const V>)(b + boff);
+  } else {
+    return (gptr<const V>)(base + o);
+  }
+}
+template <bool kOpaque, typename V>
+__device__ __forceinline__ gptr<V> at_b(V* base, int o, unsigned boff) {
+  if constexpr (kOpaque) {
+    auto b = (gptr<char>)base;
+    asm volatile("" : "+s"(b));
+    return (gptr<V>)(b + boff);
+  } else {
+    return (gptr<V>)(base + o);
+  }
+}
+
 // I is Inst (a kernel argument) or Inst in the constant address space (a
 // batched launch's descriptor): either way its fields are scalar loads.
 // S: the stage (1, 2) as a constant, or 0 for the argument stage_arg (FP64).
@@ -254,6 +286,12 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     return -1;
   };
   const int zr_ring = has_ring ? zring(r_ring % RX, r_ring / RX) : -1;
+  // byte offsets of the own and ring nodes within a plane (input, base
+  // state); each plane takes fresh copies (see the loop top)
+  const unsigned bi_own0 = (unsigned)o_own * sizeof(UIn), bi_ring0 = (unsigned)o_ring * sizeof(UIn);
+  const unsigned bd_own0 = (unsigned)o_own * sizeof(double);
+  unsigned bi_own = bi_own0, bi_ring = bi_ring0, bd_own = bd_own0;
+  constexpr bool kSaddr = sizeof(T) == 4;
   // register windows along the own column
   T Uw[5][5];       // U at planes t-4..t (index 4 = t)
   T ww[5];          // w at t-4..t
@@ -325,11 +363,13 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   }
   auto load_plane = [&](int t) {
     const long long n = (long long)t * PL;
+    unsigned bi_own = bi_own0, bi_ring = bi_ring0;
+    if constexpr (kSaddr) asm volatile("" : "+v"(bi_own), "+v"(bi_ring));
 #pragma unroll
-    for (int c = 0; c < 5; c++) Un1[c] = Uin[c * F + n + o_own];
+    for (int c = 0; c < 5; c++) Un1[c] = *at_b<kSaddr>(Uin + c * F + n, o_own, bi_own);
     if (has_ring) {
 #pragma unroll
-      for (int c = 0; c < 5; c++) Ur1[c] = Uin[c * F + n + o_ring];
+      for (int c = 0; c < 5; c++) Ur1[c] = *at_b<kSaddr>(Uin + c * F + n, o_ring, bi_ring);
     }
   };
   if constexpr (kAhead) load_plane(tb);
@@ -337,6 +377,15 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   const double tdiff = cread(a.force, 0) - cread(a.force, 1);
   for (int t = tb; t <= tend; t++) {
     const int s = t & 1;
+    // The node offsets, made opaque for this plane: otherwise the compiler
+    // folds them into loop-invariant 64-bit addresses per field (20 VGPRs)
+    // and adds the plane offset to each with a 64-bit vector add.
+    if constexpr (kSaddr) {
+      bi_own = bi_own0;
+      bi_ring = bi_ring0;
+      bd_own = bd_own0;
+      asm volatile("" : "+v"(bi_own), "+v"(bi_ring), "+v"(bd_own));
+    }
     const bool have_plane = t <= n3 - 1;
     // This iteration's metric and forcing values, loaded here and made
     // resident before the phase-1 barrier (the empty asm uses them there):
@@ -360,9 +409,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
 #pragma unroll
         for (int c = 0; c < 5; c++) Uo[c] = Un1[c];
       } else {
-        const long long n = (long long)t * PL + o_own;
+        const long long n = (long long)t * PL;
 #pragma unroll
-        for (int c = 0; c < 5; c++) Uo[c] = Uin[c * F + n];
+        for (int c = 0; c < 5; c++) Uo[c] = *at_b<kSaddr>(Uin + c * F + n, o_own, bi_own);
       }
       prim(o_own, t, u, v, w, Tt, p, mu, Uo);
       u_t = u;
@@ -394,9 +443,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
 #pragma unroll
           for (int c = 0; c < 5; c++) Uo[c] = Ur1[c];
         } else {
-          const long long n = (long long)t * PL + o_ring;
+          const long long n = (long long)t * PL;
 #pragma unroll
-          for (int c = 0; c < 5; c++) Uo[c] = Uin[c * F + n];
+          for (int c = 0; c < 5; c++) Uo[c] = *at_b<kSaddr>(Uin + c * F + n, o_ring, bi_ring);
         }
         prim(o_ring, t, u, v, w, Tt, p, mu, Uo);
         const int r = r_ring;
@@ -440,22 +489,25 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const long long n = (long long)(t + 1) * PL;
 #pragma unroll
         for (int c = 0; c < 5; c++)
-          __builtin_amdgcn_global_load_lds((const unsigned int*)(Uin + c * F + n + o_own),
-                                           &s_sink[wave * 64], 4, 0, 0);
+          __builtin_amdgcn_global_load_lds(
+              (gptr<const unsigned int>)at_b<kSaddr>(Uin + c * F + n, o_own, bi_own),
+              &s_sink[wave * 64], 4, 0, 0);
         if (has_ring) {
 #pragma unroll
           for (int c = 0; c < 5; c++)
-            __builtin_amdgcn_global_load_lds((const unsigned int*)(Uin + c * F + n + o_ring),
-                                             &s_sink[wave * 64], 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (gptr<const unsigned int>)at_b<kSaddr>(Uin + c * F + n, o_ring, bi_ring),
+                &s_sink[wave * 64], 4, 0, 0);
         }
       }
       // the base state of phase 4 (stage 2)
       if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
-        const long long nb = (long long)(t - 2) * PL + o_own;
+        const long long nb = (long long)(t - 2) * PL;
 #pragma unroll
         for (int c = 0; c < 5; c++)
-          __builtin_amdgcn_global_load_lds((const unsigned int*)(Ubase + c * F + nb),
-                                           &s_sink[wave * 64], 4, 0, 0);
+          __builtin_amdgcn_global_load_lds(
+              (gptr<const unsigned int>)at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own),
+              &s_sink[wave * 64], 4, 0, 0);
       }
     }
 
@@ -810,21 +862,23 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
       }
       if (own_valid) {
-        const long long n = (long long)q * PL + o_own;
+        const long long n = (long long)q * PL;
         double Un[5];
         if (stage == 1) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; c++) Un[c] = Ubase[c * F + n] + ph.dt * R[c];
+          for (int c = 0; c < 5; c++)
+            Un[c] = *at_b<kSaddr>(Ubase + c * F + n, o_own, bd_own) + ph.dt * R[c];
         }
         if constexpr (kOut32) {
 #pragma unroll
-          for (int c = 0; c < 5; c++) ((float*)Uout)[c * F + n] = (float)Un[c];
+          for (int c = 0; c < 5; c++)
+            *at_b<kSaddr>((float*)Uout + c * F + n, o_own, bd_own / 2) = (float)Un[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; c++) Uout[c * F + n] = Un[c];
+          for (int c = 0; c < 5; c++) *at_b<kSaddr>(Uout + c * F + n, o_own, bd_own) = Un[c];
         }
         if (stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
           // with rho > 0, T < 0 <=> rho^2 T = rho (rho E) - ggM/2 |rho u|^2 < 0
