@@ -357,6 +357,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   // which the FP32 predictor it produces absorbs.
   constexpr bool kAhead = sizeof(T) == 4;
   UIn Un1[5], Ur1[5];
+  double Ub[5];  // stage 2, mixed: base state of plane t-2
   if constexpr (kAhead) {
 #pragma unroll
     for (int c = 0; c < 5; c++) Ur1[c] = UIn(0);
@@ -500,14 +501,20 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
                 &s_sink[wave * 64], 4, 0, 0);
         }
       }
-      // the base state of phase 4 (stage 2)
+      // the base state of phase 4 (stage 2): the mixed kernel loads it into
+      // registers here, FP64 warms it in L2
       if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
         const long long nb = (long long)(t - 2) * PL;
+        if constexpr (kAhead) {
 #pragma unroll
-        for (int c = 0; c < 5; c++)
-          __builtin_amdgcn_global_load_lds(
-              (gptr<const unsigned int>)at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own),
-              &s_sink[wave * 64], 4, 0, 0);
+          for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++)
+            __builtin_amdgcn_global_load_lds(
+                (gptr<const unsigned int>)at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own),
+                &s_sink[wave * 64], 4, 0, 0);
+        }
       }
     }
 
@@ -867,6 +874,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         if (stage == 1) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
+        } else if constexpr (kAhead) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) Un[c] = Ub[c] + ph.dt * R[c];
         } else {
 #pragma unroll
           for (int c = 0; c < 5; c++)
