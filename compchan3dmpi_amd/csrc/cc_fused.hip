@@ -210,7 +210,10 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   static_assert(kRingF0 + 2 * TY <= NT, "ring passes do not fit the block");
 
   __shared__ T sl[2][4][NR];    // u, v, w, T of two planes
-  __shared__ T spm[2][2][NR];   // p, mu of two planes
+  // p and mu are read only at the node whose point fluxes are computed:
+  // the own nodes and the ring-1 points, two planes each
+  __shared__ T spo[2][2][NT];   // p, mu of the own nodes
+  __shared__ T spr[2][2][NZR];  // p, mu on the ring-1 points
   __shared__ T sub[5 * NR];     // U(5) of plane t (phase 2)
   __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
@@ -219,6 +222,14 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   // base state into L2 while phases 2 and 3 compute; the barrier after phase
   // 3 waits for them.  One 4-byte word per node and field lands here.
   __shared__ unsigned int s_sink[NT];
+  // FP64 stage 2: the base state U^n of plane t-2, brought into LDS by
+  // LDS-DMA loads issued after the phase-1 barrier and read back by phase 4
+  // (the low and high words of the five fields, lane-linear per wave: each
+  // thread reads back what its own lane loaded, so no barrier orders them).
+  // Loaded from L2 in phase 4 instead, the five loads went out one at a
+  // time (no registers to hold them) and their latency stood exposed.
+  constexpr bool kUbLds = sizeof(T) == 8 && S == 2;
+  __shared__ unsigned int s_ub[kUbLds ? 10 : 1][kUbLds ? NT : 1];
 
   const auto& g = a.g;
   const auto& ph = a.ph;
@@ -422,8 +433,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       sl[s][1][r_own] = v;
       sl[s][2][r_own] = w;
       sl[s][3][r_own] = Tt;
-      spm[s][0][r_own] = p;
-      spm[s][1][r_own] = mu;
+      spo[s][0][tid] = p;
+      spo[s][1][tid] = mu;
 #pragma unroll
       for (int c = 0; c < 5; c++) sub[c * NR + r_own] = T(Uo[c]);
       // shift the column windows
@@ -455,13 +466,13 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
           srz[t % 3][0][zr] = u;
           srz[t % 3][1][zr] = v;
           srz[t % 3][2][zr] = w;
+          spr[s][0][zr] = p;
+          spr[s][1][zr] = mu;
         }
         sl[s][0][r] = u;
         sl[s][1][r] = v;
         sl[s][2][r] = w;
         sl[s][3][r] = Tt;
-        spm[s][0][r] = p;
-        spm[s][1][r] = mu;
 #pragma unroll
         for (int c = 0; c < 5; c++) sub[c * NR + r] = T(Uo[c]);
       }
@@ -508,6 +519,13 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         if constexpr (kAhead) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
+        } else if constexpr (kUbLds) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) {
+            const auto lo = (gptr<const unsigned int>)at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
+            __builtin_amdgcn_global_load_lds(lo, &s_ub[2 * c][wave * 64], 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(lo + 1, &s_ub[2 * c + 1][wave * 64], 4, 0, 0);
+          }
         } else {
 #pragma unroll
           for (int c = 0; c < 5; c++)
@@ -598,7 +616,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     if (have_plane && t >= kb && t < ke && sizeof(T) == 4) {
       const T* U5 = sub;
       const int ro = r_own;
-      const T mu = spm[s][1][ro];
+      const T mu = mu_t;
       // x
       {
         const T ru = U5[1 * NR + ro];
@@ -661,7 +679,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       T c[4];      // u, v, w, T at r
       T pr, mu;
     };
-    auto gather = [&](int r, Sten& G) {
+    // (r: region index; pm0, pm1: the node's p and mu, own or ring-1)
+    auto gather = [&](int r, const T* pm0, const T* pm1, Sten& G) {
 #pragma unroll
       for (int f = 0; f < 4; f++) {
         const T* q = sl[sp][f];
@@ -671,8 +690,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         G.nb[f][3] = q[r + RX];
         G.c[f] = q[r];
       }
-      G.pr = spm[sp][0][r];
-      G.mu = spm[sp][1][r];
+      G.pr = *pm0;
+      G.mu = *pm1;
     };
     auto derivs_of = [&](const Sten& G, const T zm[3], const T zp[3], T zTm, T zTp, Deriv<T>& D) {
       D.dudx = (G.nb[0][1] - G.nb[0][0]) * hx;
@@ -695,7 +714,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       const T zm[3] = {zU[0], zU[1], zU[2]};
       Deriv<T> D;
       Sten G;
-      gather(r, G);
+      gather(r, &spo[sp][0][tid], &spo[sp][1][tid], G);
       const T zp[3] = {sl[s][0][r], sl[s][1][r], sl[s][2][r]};
       const T zTp = sl[s][3][r];
       __builtin_amdgcn_sched_barrier(0);
@@ -758,7 +777,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
         Deriv<T> D;
         Sten G;
-        gather(r, G);
+        gather(r, &spr[sp][0][zr], &spr[sp][1][zr], G);
         __builtin_amdgcn_sched_barrier(0);
         derivs_of(G, zm, zp, T(0), T(0), D);
         const T u = G.c[0], v = G.c[1], w = G.c[2], pr = G.pr, mu = G.mu;
@@ -792,7 +811,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
         Deriv<T> D;
         Sten G;
-        gather(r, G);
+        gather(r, &spr[sp][0][zr], &spr[sp][1][zr], G);
         __builtin_amdgcn_sched_barrier(0);
         derivs_of(G, zm, zp, T(0), T(0), D);
         const T u = G.c[0], v = G.c[1], w = G.c[2], pr = G.pr, mu = G.mu;
@@ -877,6 +896,10 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         } else if constexpr (kAhead) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = Ub[c] + ph.dt * R[c];
+        } else if constexpr (kUbLds) {
+#pragma unroll
+          for (int c = 0; c < 5; c++)
+            Un[c] = __hiloint2double((int)s_ub[2 * c + 1][tid], (int)s_ub[2 * c][tid]) + ph.dt * R[c];
         } else {
 #pragma unroll
           for (int c = 0; c < 5; c++)
@@ -902,7 +925,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         }
       }
     }
-    // the next plane overwrites sl/spm/srz slots and sub read above
+    // the next plane overwrites sl/spo/spr/srz slots and sub read above
     __syncthreads();
   }
 }
@@ -1282,9 +1305,9 @@ int choose_tiling(Inst& a, int mixed, int* ncu_out, int* bpc_out) {
 template <typename T, int TX, int TY>
 static void launch_one(const Inst& a, int stage, int step, hipStream_t s) {
   const unsigned nblk = (unsigned)(a.ntx * a.nty * a.nz);
-  // FP64: stage 2 has a kernel of its own (7 spilled VGPRs instead of 11,
-  // 0.5 % faster); a stage-1 one spilled 49, so stage 1 keeps the kernel
-  // that takes the stage as an argument
+  // FP64: stage 2 has a kernel of its own (it stages the base state in
+  // LDS); a stage-1 one spills (112 bytes at 256 VGPRs), so stage 1 keeps
+  // the kernel that takes the stage as an argument
   if constexpr (sizeof(T) == 8) {
     if (stage == 2)
       hipLaunchKernelGGL((k_stage_fused<T, TX, TY, 2>), dim3(nblk), dim3(TX * TY), 0, s, a, stage, step);
@@ -1343,7 +1366,8 @@ static int blocks_per_cu1() {
 template <typename T, int TX, int TY>
 static int blocks_per_cu() {
   if constexpr (sizeof(T) == 8) {
-    return blocks_per_cu1<T, TX, TY, 0>();
+    const int n0 = blocks_per_cu1<T, TX, TY, 0>(), n2 = blocks_per_cu1<T, TX, TY, 2>();
+    return n0 < n2 ? n0 : n2;
   } else {
     const int n1 = blocks_per_cu1<T, TX, TY, 1>(), n2 = blocks_per_cu1<T, TX, TY, 2>();
     return n1 < n2 ? n1 : n2;
@@ -1363,14 +1387,16 @@ int batched_blocks_per_cu(int mixed, int variant) {
 template <typename T, int TX, int TY>
 static void launch_b(const Inst* d_insts, const int4* d_work, int nwork, unsigned* counter,
                      unsigned base, int nblocks, int stage, int step, hipStream_t s) {
-  if constexpr (sizeof(T) == 8)
+  // FP64: stage 1 takes the stage-argument kernel, stage 2 its own (as the
+  // one-instance launch)
+  if (stage == 2)
+    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 2>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
+                       d_work, nwork, counter, base, stage, step);
+  else if constexpr (sizeof(T) == 8)
     hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 0>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
                        d_work, nwork, counter, base, stage, step);
-  else if (stage == 1)
-    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 1>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
-                       d_work, nwork, counter, base, stage, step);
   else
-    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 2>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
+    hipLaunchKernelGGL((k_stage_batched<T, TX, TY, 1>), dim3(nblocks), dim3(TX * TY), 0, s, d_insts,
                        d_work, nwork, counter, base, stage, step);
 }
 
