@@ -32,9 +32,11 @@
 // The per-plane metric and forcing values are scalar loads (cread), so no
 // vector-load wait drains the state loads in flight.  FP32 prefetches the
 // next plane into registers after phase 3 and keeps its z window in FP32,
-// which holds it to 168 VGPRs and three waves per SIMD; FP64, out of
-// registers, warms the next plane in L2 with unread LDS-DMA loads during
-// phases 2-3.  Both warm the base state U^n that phase 4 reads.
+// which holds it to 168 VGPRs and three waves per SIMD; in stage 2 it loads
+// the base state U^n of phase 4 after the phase-1 barrier.  FP64, out of
+// registers, warms the next plane in L2 with unread LDS-DMA loads, one per
+// cache line, during phases 2-3, and in stage 2 warms U^n the same way and
+// loads it after the phase-3 barrier, so that phase 3b hides the L2 hit.
 #include <type_traits>
 
 #include "cc_device.h"
@@ -217,19 +219,12 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   __shared__ T sub[5 * NR];     // U(5) of plane t (phase 2)
   __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
-  // L2 warming: LDS-DMA loads that nothing reads pull the cache lines of the
-  // next plane (FP64, which has no registers to prefetch it) and of phase 4's
-  // base state into L2 while phases 2 and 3 compute; the barrier after phase
-  // 3 waits for them.  One 4-byte word per node and field lands here.
+  // L2 warming (FP64, which has no registers to prefetch into): LDS-DMA
+  // loads that nothing reads pull the cache lines of the next plane, and in
+  // stage 2 of phase 4's base state, into L2 while phases 2 and 3 compute;
+  // the barrier after phase 3 waits for them.  One 4-byte word per lane
+  // lands here.
   __shared__ unsigned int s_sink[NT];
-  // FP64 stage 2: the base state U^n of plane t-2, brought into LDS by
-  // LDS-DMA loads issued after the phase-1 barrier and read back by phase 4
-  // (the low and high words of the five fields, lane-linear per wave: each
-  // thread reads back what its own lane loaded, so no barrier orders them).
-  // Loaded from L2 in phase 4 instead, the five loads went out one at a
-  // time (no registers to hold them) and their latency stood exposed.
-  constexpr bool kUbLds = sizeof(T) == 8 && S == 2;
-  __shared__ unsigned int s_ub[kUbLds ? 10 : 1][kUbLds ? NT : 1];
 
   const auto& g = a.g;
   const auto& ph = a.ph;
@@ -297,6 +292,61 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     return -1;
   };
   const int zr_ring = has_ring ? zring(r_ring % RX, r_ring / RX) : -1;
+  // The thread's ring-pass point, if any, fixed for the launch: an fD point
+  // on an x ring-1 column or a gD point on a y ring-1 row (region index,
+  // ring-1 slot, slot in the fD/gD array).  The x-ring points go to the wave
+  // after the y-ring ones, so that the two extra passes run on different
+  // SIMDs (waves are dealt to the SIMDs round-robin).  Computed here rather
+  // than per plane, the index keeps the LDS accesses attributable to their
+  // arrays, which lets the compiler keep LDS-DMA loads in flight past them.
+  // FP64: the next plane is warmed in L2 by one LDS-DMA load per cache line
+  // of its region (own and halo rows, five fields), not one per node: the
+  // CU moves LDS-DMA data at ~10 bytes per clock, and one load per node and
+  // field took ~8 % of the plane time.  The probes lie at most 16 nodes (one
+  // 128-byte line) apart along each of the TY + 4 rows, both ends included;
+  // threads past the probe count idle.
+  constexpr int kProbeX = (TX + 4 + 15) / 16 + 1, kProbeRow = (TY + 4) * kProbeX;
+  static_assert(sizeof(T) == 4 || 5 * kProbeRow <= NT, "one probe per thread");
+  bool warm_on = false;
+  long long warm_e = 0;  // field and node, as an element offset within a plane
+  if constexpr (sizeof(T) == 8) {
+    if (tid < 5 * kProbeRow) {
+      const int c = tid / kProbeRow, row = (tid % kProbeRow) / kProbeX, k = tid % kProbeX;
+      const int col = k == kProbeX - 1 ? TX + 1 : 16 * k - 2;  // relative to i0
+      warm_e = c * F + (long long)wrapP(j0 + row - 2, P2) * P1 + wrapP(i0 + col, P1);
+      warm_on = true;
+    }
+  }
+  // FP64 stage 2: the same for the base state's lines (own rows only)
+  constexpr int kBProbeX = (TX + 15) / 16 + 1, kBProbe = 5 * TY * kBProbeX;
+  static_assert(sizeof(T) == 4 || kBProbe <= NT, "one probe per thread");
+  bool bwarm_on = false;
+  long long bwarm_e = 0;
+  if constexpr (sizeof(T) == 8) {
+    if (tid < kBProbe) {
+      const int c = tid / (TY * kBProbeX), row = (tid % (TY * kBProbeX)) / kBProbeX,
+                k = tid % kBProbeX;
+      const int col = k == kBProbeX - 1 ? TX - 1 : 16 * k;  // relative to i0
+      bwarm_on = j0 + row < P2 && i0 + col < P1;
+      bwarm_e = c * F + (long long)min(j0 + row, P2 - 1) * P1 + min(i0 + col, P1 - 1);
+    }
+  }
+  const int rf_t = tid - kRingF0, rg_t = tid - kRingG0;
+  const bool fring = rf_t >= 0 && rf_t < 2 * TY, gring = rg_t >= 0 && rg_t < 2 * TX;
+  int rp_r = 0, rp_zr = 0, rp_li = 0;
+  if (fring) {
+    const int side = rf_t / TY, yy = rf_t % TY;
+    const int rx = side ? TX + 2 : 1, ry = yy + 2;
+    rp_r = ry * RX + rx;
+    rp_zr = zring(rx, ry);
+    rp_li = yy * (TX + 2) + (rx - 1);
+  } else if (gring) {
+    const int side = rg_t / TX, xx = rg_t % TX;
+    const int ry = side ? TY + 2 : 1, rx = xx + 2;
+    rp_r = ry * RX + rx;
+    rp_zr = zring(rx, ry);
+    rp_li = (ry - 1) * TX + xx;
+  }
   // byte offsets of the own and ring nodes within a plane (input, base
   // state); each plane takes fresh copies (see the loop top)
   const unsigned bi_own0 = (unsigned)o_own * sizeof(UIn), bi_ring0 = (unsigned)o_ring * sizeof(UIn);
@@ -362,13 +412,17 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   // The mixed kernel has registers to spare (FP64 runs at 256): right after
   // the barrier that ends phase 3 it loads the next plane's state, so that
   // phases 3b and 4 hide its latency.  FP64 loads it where it is used.
-  // Stage 2 of both reads its base state U^n in phase 4, from L2 (warmed
-  // after the phase-1 barrier).  Stage 1 takes it from the z window: FP64
-  // exactly; in the mixed kernel, whose window is FP32, rounded to FP32,
-  // which the FP32 predictor it produces absorbs.
+  // Stage 2 reads its base state U^n into Ub: the mixed kernel after the
+  // phase-1 barrier, FP64 after the phase-3 barrier (from L2, warmed after
+  // the phase-1 barrier; held longer, it would not fit in 256 VGPRs).
+  // Stage 1 takes it from the z window: FP64 exactly; in the mixed kernel,
+  // whose window is FP32, rounded to FP32, which the FP32 predictor it
+  // produces absorbs.  The FP64 stage-argument kernel (stage 1 only, in
+  // practice) loads it in phase 4 when run for stage 2.
   constexpr bool kAhead = sizeof(T) == 4;
+  constexpr bool kUbLate = sizeof(T) == 8 && S == 2;
   UIn Un1[5], Ur1[5];
-  double Ub[5];  // stage 2, mixed: base state of plane t-2
+  double Ub[5];  // stage 2: base state of plane t-2
   if constexpr (kAhead) {
 #pragma unroll
     for (int c = 0; c < 5; c++) Ur1[c] = UIn(0);
@@ -497,41 +551,23 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     __syncthreads();
     {
       const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-      if (!kAhead && t + 1 <= tlast) {  // plane t+1: own and ring nodes (FP64)
-        const long long n = (long long)(t + 1) * PL;
-#pragma unroll
-        for (int c = 0; c < 5; c++)
-          __builtin_amdgcn_global_load_lds(
-              (gptr<const unsigned int>)at_b<kSaddr>(Uin + c * F + n, o_own, bi_own),
-              &s_sink[wave * 64], 4, 0, 0);
-        if (has_ring) {
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            __builtin_amdgcn_global_load_lds(
-                (gptr<const unsigned int>)at_b<kSaddr>(Uin + c * F + n, o_ring, bi_ring),
-                &s_sink[wave * 64], 4, 0, 0);
-        }
-      }
+      if (!kAhead && t + 1 <= tlast && warm_on)  // plane t+1 into L2 (FP64)
+        __builtin_amdgcn_global_load_lds(
+            (gptr<const unsigned int>)(Uin + (long long)(t + 1) * PL + warm_e), &s_sink[wave * 64],
+            4, 0, 0);
       // the base state of phase 4 (stage 2): the mixed kernel loads it into
-      // registers here, FP64 warms it in L2
-      if (stage == 2 && t - 2 >= kb && t - 2 < ke && own_valid) {
+      // registers here, FP64 warms its lines in L2
+      if (stage == 2 && t - 2 >= kb && t - 2 < ke) {
         const long long nb = (long long)(t - 2) * PL;
         if constexpr (kAhead) {
+          if (own_valid) {
 #pragma unroll
-          for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
-        } else if constexpr (kUbLds) {
-#pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const auto lo = (gptr<const unsigned int>)at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
-            __builtin_amdgcn_global_load_lds(lo, &s_ub[2 * c][wave * 64], 4, 0, 0);
-            __builtin_amdgcn_global_load_lds(lo + 1, &s_ub[2 * c + 1][wave * 64], 4, 0, 0);
+            for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
           }
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            __builtin_amdgcn_global_load_lds(
-                (gptr<const unsigned int>)at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own),
-                &s_sink[wave * 64], 4, 0, 0);
+        } else if constexpr (kUbLate) {
+          if (bwarm_on)
+            __builtin_amdgcn_global_load_lds((gptr<const unsigned int>)(Ubase + nb + bwarm_e),
+                                             &s_sink[wave * 64], 4, 0, 0);
         }
       }
     }
@@ -763,15 +799,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
 #pragma unroll
         for (int c = 0; c < 4; c++) fx[c * NFX + li] = Fx[c];
       }
-      // the x-ring points go to the wave after the y-ring ones, so that the
-      // two extra passes run on different SIMDs (waves are dealt to the
-      // SIMDs round-robin) instead of both on wave 0
-      const int rf = tid - kRingF0;
-      if (rf >= 0 && rf < 2 * TY) {
-        const int side = rf / TY, yy = rf % TY;
-        const int rx = side ? TX + 2 : 1, ry = yy + 2;
-        const int r = ry * RX + rx;
-        const int zr = zring(rx, ry);
+      if (fring) {
+        const int r = rp_r, zr = rp_zr;
         const int tm = (p - 1 + 3) % 3, tp = (p + 1) % 3;
         const T zm[3] = {srz[tm][0][zr], srz[tm][1][zr], srz[tm][2][zr]};
         const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
@@ -785,7 +814,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const T mR = mu * invRe;
         const T sxz = D.dwdx + D.dudz, sxy = D.dvdx + D.dudy;
         const T dissF = ggM * invRe * mu * (t23 * u * divg - T(2) * u * D.dudx - v * sxy - w * sxz);
-        const int li = yy * (TX + 2) + (rx - 1);
+        const int li = rp_li;
         fx[0 * NFX + li] = pr + t23 * mR * divg - T(2) * mR * D.dudx;
         fx[1 * NFX + li] = -mR * sxy;
         fx[2 * NFX + li] = -mR * sxz;
@@ -800,12 +829,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
 #pragma unroll
         for (int c = 0; c < 4; c++) gy[c * NGY + li] = Gy[c];
       }
-      const int rg = tid - kRingG0;
-      if (rg >= 0 && rg < 2 * TX) {
-        const int side = rg / TX, xx = rg % TX;
-        const int ry = side ? TY + 2 : 1, rx = xx + 2;
-        const int r = ry * RX + rx;
-        const int zr = zring(rx, ry);
+      if (gring) {
+        const int r = rp_r, zr = rp_zr;
         const int tm = (p - 1 + 3) % 3, tp = (p + 1) % 3;
         const T zm[3] = {srz[tm][0][zr], srz[tm][1][zr], srz[tm][2][zr]};
         const T zp[3] = {srz[tp][0][zr], srz[tp][1][zr], srz[tp][2][zr]};
@@ -819,7 +844,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         const T mR = mu * invRe;
         const T syz = D.dwdy + D.dvdz, sxy = D.dvdx + D.dudy;
         const T dissG = ggM * invRe * mu * (-u * sxy + t23 * v * divg - T(2) * v * D.dvdy - w * syz);
-        const int li = (ry - 1) * TX + xx;
+        const int li = rp_li;
         gy[0 * NGY + li] = -mR * sxy;
         gy[1 * NGY + li] = pr + t23 * mR * divg - T(2) * mR * D.dvdy;
         gy[2 * NGY + li] = -mR * syz;
@@ -829,6 +854,13 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     __syncthreads();
     if constexpr (kAhead) {
       if (t + 1 <= tlast) load_plane(t + 1);
+    }
+    if constexpr (kUbLate) {
+      if (t - 2 >= kb && t - 2 < ke && own_valid) {
+        const long long nq = (long long)(t - 2) * PL;
+#pragma unroll
+        for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nq, o_own, bd_own);
+      }
     }
     if (need_fg) {
       const T* fx = sfg;
@@ -893,13 +925,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         if (stage == 1) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
-        } else if constexpr (kAhead) {
+        } else if constexpr (kAhead || kUbLate) {
 #pragma unroll
           for (int c = 0; c < 5; c++) Un[c] = Ub[c] + ph.dt * R[c];
-        } else if constexpr (kUbLds) {
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            Un[c] = __hiloint2double((int)s_ub[2 * c + 1][tid], (int)s_ub[2 * c][tid]) + ph.dt * R[c];
         } else {
 #pragma unroll
           for (int c = 0; c < 5; c++)
