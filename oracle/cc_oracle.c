@@ -289,7 +289,11 @@ int orc_load(orc_t *o, const double *v, int nstep, double time) {
 
 /* ---- convective flux: main:329-454 -------------------------------------- */
 /* QUICK face pair of main:339-361 for one component: upwind-biased face
- * values at e (between c and p) and w (between m and c). */
+ * values at e (between c and p) and w (between m and c).  With a NaN face
+ * velocity none of the reference's four cases applies and its F keeps the
+ * value of the previous evaluation (F is a persistent array); this returns
+ * 0 there.  The two agree on every state without NaN; how NaN spreads after
+ * its first appearance is parity unpinned. */
 static inline double quick(double ue, double uw, double qmm, double qm,
                            double qc, double qp, double qpp) {
   double f = 0.0;
