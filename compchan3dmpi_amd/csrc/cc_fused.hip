@@ -1017,10 +1017,13 @@ k_stage_batched(const Inst* __restrict__ insts, const int4* __restrict__ work, i
 // threads on the domain edges also update the wall's periodic duplicates
 // (the reference's boundary loop covers them at their own coordinates) and
 // the four corners, whose interior columns the reference never refreshes.
-// One 64 x 4 block of wall nodes.  The block first recovers the primitives of
-// its patch plus a 1-node x/y halo at the two planes next to the wall, once per
-// column, into LDS; each node then reads the five columns it needs from there.
-constexpr int kWX = 64, kWY = 4, kWLX = kWX + 2, kWLY = kWY + 2;
+// One 64 x 4 block of wall nodes, run by 64 x 8 threads.  The block first
+// recovers the primitives of its patch plus a 1-node x/y halo at the two
+// planes next to the wall, up to two columns per thread, into LDS; the threads of
+// the first four rows then compute a wall node each from the five columns it
+// needs.  (With 256 threads, each recovered up to four columns in turn, at
+// one wave per SIMD.)
+constexpr int kWX = 64, kWY = 4, kWTY = 8, kWLX = kWX + 2, kWLY = kWY + 2;
 __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, int wall) {
   __shared__ double wq[7][2][kWLY][kWLX];  // r, u, v, w, T, p, mu
   const Geom& g = a.g;
@@ -1058,7 +1061,7 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
   struct P {
     double r, u, v, w, T, p, mu;
   };
-  const bool valid = ii < g.P1 && jj < g.P2;
+  const bool valid = threadIdx.y < kWY && ii < g.P1 && jj < g.P2;
   const double pold = valid ? pw[wall * PL + n2d] : 0.0;
   {
     // the patch's columns at planes k1, k2 (its four corners are not needed),
@@ -1067,7 +1070,7 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
       x %= P;
       return x < 0 ? x + P : x;
     };
-    constexpr int NE = 2 * kWLX * kWLY, NT = kWX * kWY, NPER = (NE + NT - 1) / NT;
+    constexpr int NE = 2 * kWLX * kWLY, NT = kWX * kWTY, NPER = (NE + NT - 1) / NT;
     const int tid = threadIdx.y * kWX + threadIdx.x;
     double Uq[NPER][5];
     bool use[NPER];
@@ -1096,8 +1099,9 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
       const int e = tid + m * NT;
       const int pl = e / (kWLX * kWLY), cc = e % (kWLX * kWLY);
       const int ly = cc / kWLX, lx = cc % kWLX;
+      // the stage kernel's arithmetic (reciprocal, 1/gM2)
       const double r = Uq[m][0];
-      const double ir = 1.0 / r;
+      const double ir = rcp<double>(r);
       const double u = Uq[m][1] * ir, v = Uq[m][2] * ir, w = Uq[m][3] * ir;
       const double T = Uq[m][4] * ir - ph.ggM * 0.5 * (u * u + v * v + w * w);
       wq[0][pl][ly][lx] = r;
@@ -1105,7 +1109,7 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
       wq[2][pl][ly][lx] = v;
       wq[3][pl][ly][lx] = w;
       wq[4][pl][ly][lx] = T;
-      wq[5][pl][ly][lx] = r * T / ph.gM2;
+      wq[5][pl][ly][lx] = r * T * (1.0 / ph.gM2);
       wq[6][pl][ly][lx] = pow07d(T);  // as the stage kernel evaluates mu at these nodes
     }
   }
@@ -1217,12 +1221,12 @@ __device__ __forceinline__ void wall_tile(const Inst& a, int stage, int step, in
   pw[wall * PL + n2d] = -(lhs + c1 * c1p.p + c2 * c2p.p) / c0;
 }
 
-__global__ void __launch_bounds__(256) k_wall_fast(Inst a, int stage, int step) {
+__global__ void __launch_bounds__(512) k_wall_fast(Inst a, int stage, int step) {
   wall_tile(a, stage, step, blockIdx.z);
 }
 
 // all instances of a batch: blockIdx.z = 2 * instance + wall
-__global__ void __launch_bounds__(256) k_wall_batched(const Inst* __restrict__ insts, int stage,
+__global__ void __launch_bounds__(512) k_wall_batched(const Inst* __restrict__ insts, int stage,
                                                       int step) {
   wall_tile(insts[blockIdx.z >> 1], stage, step, blockIdx.z & 1);
 }
@@ -1361,7 +1365,7 @@ void launch_predictor_f32(const Inst& a, hipStream_t s) {
 }
 
 void launch_wall_fast(const Inst& a, int stage, int step, hipStream_t s) {
-  dim3 blk(64, 4);
+  dim3 blk(kWX, kWTY);
   dim3 grd((a.g.P1 + 63) / 64, (a.g.P2 + 3) / 4, 2);
   hipLaunchKernelGGL(k_wall_fast, grd, blk, 0, s, a, stage, step);
 }
@@ -1447,7 +1451,7 @@ void launch_stage_batched(const Inst* d_insts, const int4* d_work, int nwork, un
   }
 #undef CC_LAUNCH_B
   if (ev1) (void)hipEventRecord(ev1, s);
-  dim3 blk(64, 4);
+  dim3 blk(kWX, kWTY);
   dim3 grd((maxP1 + 63) / 64, (maxP2 + 3) / 4, 2 * ninst);
   hipLaunchKernelGGL(k_wall_batched, grd, blk, 0, s, d_insts, stage, step);
 }
