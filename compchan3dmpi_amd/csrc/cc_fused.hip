@@ -700,11 +700,14 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     const int sp = p & 1;
     const bool need_h = p >= kb - 1 && p <= ke && p >= 0;
     const bool need_fg = p >= kb && p < ke;
-    // z derivative of a ring-1 or own node at plane p
-    auto zder = [&](T qm, T q0, T qp) -> T {
-      if (p == 0) return (qp - q0) * T(ph.invdz1);
-      if (p == n3 - 1) return (q0 - qm) * T(ph.invdzN);
-      return (qp - qm) * (half * invdXi * T(iJ_p));
+    // z derivatives of a ring-1 or own node at plane p: the central form,
+    // then, on a wall plane only, the one-sided one (a single uniform branch
+    // per node pass; a branch per derivative made a chain of some 40 scalar
+    // branches per plane)
+    const T czc = half * invdXi * T(iJ_p);
+    const bool pwall = (p == 0) | (p == n3 - 1);
+    auto zder_wall = [&](T qm, T q0, T qp) -> T {
+      return p == 0 ? (qp - q0) * T(ph.invdz1) : (q0 - qm) * T(ph.invdzN);
     };
     // derivatives at a region node of plane p; z values supplied.  The
     // node's whole x/y stencil (u, v, w, T at r, r+-1, r+-RX) and its p, mu
@@ -738,10 +741,16 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       D.dvdy = (G.nb[1][3] - G.nb[1][2]) * hy;
       D.dwdy = (G.nb[2][3] - G.nb[2][2]) * hy;
       D.dTdy = (G.nb[3][3] - G.nb[3][2]) * hy;
-      D.dudz = zder(zm[0], G.c[0], zp[0]);
-      D.dvdz = zder(zm[1], G.c[1], zp[1]);
-      D.dwdz = zder(zm[2], G.c[2], zp[2]);
-      D.dTdz = zder(zTm, G.c[3], zTp);
+      D.dudz = (zp[0] - zm[0]) * czc;
+      D.dvdz = (zp[1] - zm[1]) * czc;
+      D.dwdz = (zp[2] - zm[2]) * czc;
+      D.dTdz = (zTp - zTm) * czc;
+      if (pwall) {
+        D.dudz = zder_wall(zm[0], G.c[0], zp[0]);
+        D.dvdz = zder_wall(zm[1], G.c[1], zp[1]);
+        D.dwdz = zder_wall(zm[2], G.c[2], zp[2]);
+        D.dTdz = zder_wall(zTm, G.c[3], zTp);
+      }
     };
     T Fx[4] = {T(0), T(0), T(0), T(0)}, Gy[4] = {T(0), T(0), T(0), T(0)};
     if (need_h) {
