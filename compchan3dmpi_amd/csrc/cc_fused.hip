@@ -776,16 +776,17 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       h[3] = -gkRP * mu * D.dTdz + ggM * w * pr + dissH;
       // hD(p) enters the z divergence of plane p-1 (finished by this
       // iteration's phase 4: Sg[0]) and of plane p+1 (begun in this
-      // iteration: Sg[2]); no plane of hD needs to be kept
-      if (p - 1 >= kb && p - 1 < ke) {
-        const T cm = half * invdXi * T(iJ_q);
+      // iteration: Sg[2]); no plane of hD needs to be kept.  Added without
+      // testing that those planes are in the chunk: the sums of planes
+      // outside it are never used, and a test here would split the pass
+      // into basic blocks the scheduler cannot cross.
+      {
+        const T cm = half * invdXi * T(iJ_q), cp = half * invdXi * T(iJ_t);
 #pragma unroll
-        for (int c = 0; c < 4; c++) Sg[0][c + 1] += h[c] * cm;
-      }
-      if (p + 1 >= kb && p + 1 < ke) {
-        const T cp = half * invdXi * T(iJ_t);
-#pragma unroll
-        for (int c = 0; c < 4; c++) Sg[2][c + 1] -= h[c] * cp;
+        for (int c = 0; c < 4; c++) {
+          Sg[0][c + 1] += h[c] * cm;
+          Sg[2][c + 1] -= h[c] * cp;
+        }
       }
       if (need_fg) {
         const T dissF = ggM * invRe * mu * (t23 * u * divg - T(2) * u * D.dudx - v * sxy - w * sxz);
@@ -924,10 +925,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       for (int c = 0; c < 5; c++) R[c] = -(double)(Sg[0][c] + H[c]);
       R[1] += src1;
       R[4] += src4;
-      if (ph.Ri != 0.0) {  // buoyancy -Ri rho along z and its work
-        R[3] += -ph.Ri * (double)Uw[2][0];
-        R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
-      }
+      // buoyancy -Ri rho along z and its work (adds -0 when Ri = 0: no branch)
+      R[3] += -ph.Ri * (double)Uw[2][0];
+      R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
       if (own_valid) {
         const long long n = (long long)q * PL;
         double Un[5];
