@@ -577,10 +577,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     // state coming from the z window and phase 1's registers, and only then
     // computes: the reads go out back to back and their latencies overlap,
     // instead of one LDS round trip per stencil point (sched_barrier keeps
-    // the compiler from sinking the reads back to their uses).  FP64 only:
-    // the FP32 kernels, at three waves per SIMD, hide the latency with their
-    // third wave and spill under the gathered registers.
-    if (have_plane && t >= kb && t < ke && sizeof(T) == 8) {
+    // the compiler from sinking the reads back to their uses).
+    if (have_plane && t >= kb && t < ke) {
       const T* U5 = sub;
       const int ro = r_own;
       // x
@@ -645,49 +643,6 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
                        (invdy * T(0.125));
           const T fc = (-vp2 * qy[c][3] + T(8) * vp1 * qy[c][2] - T(8) * vm1 * qy[c][1] +
                         vm2 * qy[c][0]) * (invdy * T(kOne12th));
-          Sg[2][c] += quick ? fq : fc;
-        }
-      }
-    }
-    if (have_plane && t >= kb && t < ke && sizeof(T) == 4) {
-      const T* U5 = sub;
-      const int ro = r_own;
-      const T mu = mu_t;
-      // x
-      {
-        const T ru = U5[1 * NR + ro];
-        const T* ul = sl[s][0];
-        if (T(ph.Re) * fabs(ru) * T(ph.dx) > T(ph.Pex) * mu) {
-          const T t1 = (ul[ro] + ul[ro + 1]) * half, t2 = (ul[ro] + ul[ro - 1]) * half;
-#pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const T* q = U5 + c * NR + ro;
-            Sg[2][c] += quickT<T>(t1, t2, q[-2], q[-1], q[0], q[1], q[2]) * (invdx * T(0.125));
-          }
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const T* q = U5 + c * NR + ro;
-            Sg[2][c] += (-ul[ro + 2] * q[2] + T(8) * ul[ro + 1] * q[1] - T(8) * ul[ro - 1] * q[-1] +
-                         ul[ro - 2] * q[-2]) * (invdx * T(kOne12th));
-          }
-        }
-      }
-      // y: both schemes from one read of the stencil, the node's own selected
-      {
-        const T rv = U5[2 * NR + ro];
-        const T* vl = sl[s][1];
-        const bool quick = T(ph.Re) * fabs(rv) * T(ph.dy) > T(ph.Pey) * mu;
-        const T vm2 = vl[ro - 2 * RX], vm1 = vl[ro - RX], v0 = vl[ro], vp1 = vl[ro + RX],
-                vp2 = vl[ro + 2 * RX];
-        const T t1 = (v0 + vp1) * half, t2 = (v0 + vm1) * half;
-#pragma unroll
-        for (int c = 0; c < 5; c++) {
-          const T* q = U5 + c * NR + ro;
-          const T qm2 = q[-2 * RX], qm1 = q[-RX], q0 = q[0], qp1 = q[RX], qp2 = q[2 * RX];
-          const T fq = quickT<T>(t1, t2, qm2, qm1, q0, qp1, qp2) * (invdy * T(0.125));
-          const T fc = (-vp2 * qp2 + T(8) * vp1 * qp1 - T(8) * vm1 * qm1 + vm2 * qm2) *
-                       (invdy * T(kOne12th));
           Sg[2][c] += quick ? fq : fc;
         }
       }
