@@ -917,8 +917,12 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         }
       }
     }
-    // the next plane overwrites sl/spo/spr/srz slots and sub read above
-    __syncthreads();
+    // No barrier here.  The next plane's phase 1 writes sl, spo, spr, srz
+    // and sub, which this plane last read in phases 2 and 3, before the
+    // barrier after phase 3; phases 3b and 4 read only sfg and registers,
+    // and sfg is next written in phase 3, after the next phase-1 barrier.
+    // So a wave that finishes phase 4 early starts the next plane's loads
+    // while the others finish theirs.
   }
 }
 
