@@ -421,6 +421,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   // practice) loads it in phase 4 when run for stage 2.
   constexpr bool kAhead = sizeof(T) == 4;
   constexpr bool kUbLate = sizeof(T) == 8 && S == 2;
+  constexpr bool kP4Early = sizeof(T) == 4;  // phase 4 before the barrier after phase 3
   UIn Un1[5], Ur1[5];
   double Ub[5];  // stage 2: base state of plane t-2
   if constexpr (kAhead) {
@@ -816,6 +817,86 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         gy[3 * NGY + li] = -gkRP * mu * D.dTdy + ggM * v * pr + dissG;
       }
     }
+    // ---- phase 4: complete plane q = t-2 ------------------------------
+    // It needs no other thread's values: the window, the own column's flux
+    // sums (Sg[0] is complete once this thread's phase 3 has added hD of
+    // plane t-1) and the base state.  The FP32 kernels run it before the
+    // barrier after phase 3, where waves that finish phase 3 early would
+    // otherwise wait (stage 1 6 % faster).  FP64 runs it after phase 3b: its
+    // stage-2 base state is loaded after that barrier, and its stage 1 ran
+    // 0.8 % slower with phase 4 moved.
+    auto phase4 = [&]() {
+      const int q = t - 2;
+      if (q >= kb && q < ke) {
+        const T iJ = T(iJ_q);
+        // z convective (main_v1.9.f90:409-450); column values at q-2..q+2 are
+        // window slots 0..4 (slot 2 = q)
+        T H[5];
+        const bool lowhigh = (q + 1 < 3) || (q + 1 > n3 - 2);
+        if (lowhigh) {
+  #pragma unroll
+          for (int c = 0; c < 5; c++) H[c] = (ww[3] * Uw[3][c] - ww[1] * Uw[1][c]) * (invdXi * iJ * half);
+        } else if (zq & 4u) {
+          const T t1 = (ww[2] + ww[3]) * half, t2 = (ww[2] + ww[1]) * half;
+  #pragma unroll
+          for (int c = 0; c < 5; c++)
+            H[c] = quickT<T>(t1, t2, Uw[0][c], Uw[1][c], Uw[2][c], Uw[3][c], Uw[4][c]) *
+                   (invdXi * iJ * T(0.125));
+        } else {
+  #pragma unroll
+          for (int c = 0; c < 5; c++)
+            H[c] = (-ww[4] * Uw[4][c] + T(8) * ww[3] * Uw[3][c] - T(8) * ww[1] * Uw[1][c] +
+                    ww[0] * Uw[0][c]) * (invdXi * iJ * T(kOne12th));
+        }
+        // (the z divergence of hD is already in Sg[0])
+        double R[5];
+        const double rb = rb_q;
+        const double src1 = 0.5 * tdiff * rb;
+        // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
+        const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
+  #pragma unroll
+        for (int c = 0; c < 5; c++) R[c] = -(double)(Sg[0][c] + H[c]);
+        R[1] += src1;
+        R[4] += src4;
+        // buoyancy -Ri rho along z and its work (adds -0 when Ri = 0: no branch)
+        R[3] += -ph.Ri * (double)Uw[2][0];
+        R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
+        if (own_valid) {
+          const long long n = (long long)q * PL;
+          double Un[5];
+          if (stage == 1) {
+  #pragma unroll
+            for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
+          } else if constexpr (kAhead || kUbLate) {
+  #pragma unroll
+            for (int c = 0; c < 5; c++) Un[c] = Ub[c] + ph.dt * R[c];
+          } else {
+  #pragma unroll
+            for (int c = 0; c < 5; c++)
+              Un[c] = *at_b<kSaddr>(Ubase + c * F + n, o_own, bd_own) + ph.dt * R[c];
+          }
+          if constexpr (kOut32) {
+  #pragma unroll
+            for (int c = 0; c < 5; c++)
+              *at_b<kSaddr>((float*)Uout + c * F + n, o_own, bd_own / 2) = (float)Un[c];
+          } else {
+  #pragma unroll
+            for (int c = 0; c < 5; c++) *at_b<kSaddr>(Uout + c * F + n, o_own, bd_own) = Un[c];
+          }
+          if (stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
+            // with rho > 0, T < 0 <=> rho^2 T = rho (rho E) - ggM/2 |rho u|^2 < 0
+            const double r2T = Un[0] * Un[4] - ph.ggM * 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]);
+            if (Un[0] < 0.0 || r2T < 0.0) {
+              const unsigned long long lin =
+                  ((unsigned long long)q * a.n2 + (unsigned long long)(gj_own + 1)) * a.n1 + (gi_own + 1);
+              atomicMin(a.blow, lin);
+              atomicCAS(a.halt, 0, step);
+            }
+          }
+        }
+      }
+    };
+    if constexpr (kP4Early) phase4();
     __syncthreads();
     if constexpr (kAhead) {
       if (t + 1 <= tlast) load_plane(t + 1);
@@ -847,76 +928,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         Sg[1][c + 1] += (d[c][1] - d[c][0]) * hx + (d[c][3] - d[c][2]) * hy;
     }
 
-    // ---- phase 4: complete plane q = t-2 ------------------------------
-    const int q = t - 2;
-    if (q >= kb && q < ke) {
-      const T iJ = T(iJ_q);
-      // z convective (main_v1.9.f90:409-450); column values at q-2..q+2 are
-      // window slots 0..4 (slot 2 = q)
-      T H[5];
-      const bool lowhigh = (q + 1 < 3) || (q + 1 > n3 - 2);
-      if (lowhigh) {
-#pragma unroll
-        for (int c = 0; c < 5; c++) H[c] = (ww[3] * Uw[3][c] - ww[1] * Uw[1][c]) * (invdXi * iJ * half);
-      } else if (zq & 4u) {
-        const T t1 = (ww[2] + ww[3]) * half, t2 = (ww[2] + ww[1]) * half;
-#pragma unroll
-        for (int c = 0; c < 5; c++)
-          H[c] = quickT<T>(t1, t2, Uw[0][c], Uw[1][c], Uw[2][c], Uw[3][c], Uw[4][c]) *
-                 (invdXi * iJ * T(0.125));
-      } else {
-#pragma unroll
-        for (int c = 0; c < 5; c++)
-          H[c] = (-ww[4] * Uw[4][c] + T(8) * ww[3] * Uw[3][c] - T(8) * ww[1] * Uw[1][c] +
-                  ww[0] * Uw[0][c]) * (invdXi * iJ * T(kOne12th));
-      }
-      // (the z divergence of hD is already in Sg[0])
-      double R[5];
-      const double rb = rb_q;
-      const double src1 = 0.5 * tdiff * rb;
-      // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
-      const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
-#pragma unroll
-      for (int c = 0; c < 5; c++) R[c] = -(double)(Sg[0][c] + H[c]);
-      R[1] += src1;
-      R[4] += src4;
-      // buoyancy -Ri rho along z and its work (adds -0 when Ri = 0: no branch)
-      R[3] += -ph.Ri * (double)Uw[2][0];
-      R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
-      if (own_valid) {
-        const long long n = (long long)q * PL;
-        double Un[5];
-        if (stage == 1) {
-#pragma unroll
-          for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
-        } else if constexpr (kAhead || kUbLate) {
-#pragma unroll
-          for (int c = 0; c < 5; c++) Un[c] = Ub[c] + ph.dt * R[c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            Un[c] = *at_b<kSaddr>(Ubase + c * F + n, o_own, bd_own) + ph.dt * R[c];
-        }
-        if constexpr (kOut32) {
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            *at_b<kSaddr>((float*)Uout + c * F + n, o_own, bd_own / 2) = (float)Un[c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 5; c++) *at_b<kSaddr>(Uout + c * F + n, o_own, bd_own) = Un[c];
-        }
-        if (stage == 2) {  // blow-up check on the new state (main_v1.9.f90:728-740)
-          // with rho > 0, T < 0 <=> rho^2 T = rho (rho E) - ggM/2 |rho u|^2 < 0
-          const double r2T = Un[0] * Un[4] - ph.ggM * 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]);
-          if (Un[0] < 0.0 || r2T < 0.0) {
-            const unsigned long long lin =
-                ((unsigned long long)q * a.n2 + (unsigned long long)(gj_own + 1)) * a.n1 + (gi_own + 1);
-            atomicMin(a.blow, lin);
-            atomicCAS(a.halt, 0, step);
-          }
-        }
-      }
-    }
+    if constexpr (!kP4Early) phase4();
     // No barrier here.  The next plane's phase 1 writes sl, spo, spr, srz
     // and sub, which this plane last read in phases 2 and 3, before the
     // barrier after phase 3; phases 3b and 4 read only sfg and registers,
