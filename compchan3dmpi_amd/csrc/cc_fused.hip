@@ -44,14 +44,44 @@
 
 namespace cc {
 
+// The convective flux difference at a node is linear in the five stencil
+// values (qmm, qm, qc, qp, qpp) along the direction, with weights that
+// depend only on the velocities, so the weights are formed once per node and
+// every field costs five FMAs.
+//
+// QUICK (main_v1.9.f90:335-358), scaled by s: ue fe - uw fw, where the east
+// face takes its upwind-biased value from the side ue blows from,
+//   fe = 6 qc + 3 qp - qm (ue >= 0), 6 qp + 3 qc - qpp (ue < 0),
+// and the west face from the side uw blows from,
+//   fw = 6 qm + 3 qc - qmm (uw >= 0), 6 qc + 3 qm - qp (uw < 0).
 template <typename T>
-__device__ __forceinline__ T quickT(T ue, T uw, T qmm, T qm, T qc, T qp, T qpp) {
-  // face-upwinded QUICK pair (main_v1.9.f90:339-361): the east face takes
-  // its upwind-biased value from the side ue blows from, the west face from
-  // the side uw blows from.
-  const T fe = (ue >= T(0)) ? (T(6) * qc + T(3) * qp - qm) : (T(6) * qp + T(3) * qc - qpp);
-  const T fw = (uw >= T(0)) ? (T(6) * qm + T(3) * qc - qmm) : (T(6) * qc + T(3) * qm - qp);
-  return ue * fe - uw * fw;
+__device__ __forceinline__ void quick_w(T ue, T uw, T s, T (&c)[5]) {
+  const bool ep = ue >= T(0), wp = uw >= T(0);
+  const T e = ue * s, w = uw * s;
+  c[0] = wp ? w : T(0);
+  c[1] = (ep ? -e : T(0)) - (wp ? T(6) * w : T(3) * w);
+  c[2] = (ep ? T(6) * e : T(3) * e) - (wp ? T(3) * w : T(6) * w);
+  c[3] = (ep ? T(3) * e : T(6) * e) + (wp ? T(0) : w);
+  c[4] = ep ? T(0) : -e;
+}
+// fourth-order central (main_v1.9.f90:360-366), scaled by k:
+//   -vpp qpp + 8 vp qp - 8 vm qm + vmm qmm, with the velocities vmm..vpp at
+//   the stencil points
+template <typename T>
+__device__ __forceinline__ void central_w(T vmm, T vm, T vp, T vpp, T k, T (&c)[5]) {
+  c[0] = vmm * k;
+  c[1] = T(-8) * vm * k;
+  c[2] = T(0);
+  c[3] = T(8) * vp * k;
+  c[4] = -vpp * k;
+}
+template <typename T>
+__device__ __forceinline__ T apply_w(const T (&c)[5], T qmm, T qm, T qc, T qp, T qpp, T acc) {
+  acc = fma(c[0], qmm, acc);
+  acc = fma(c[1], qm, acc);
+  acc = fma(c[2], qc, acc);
+  acc = fma(c[3], qp, acc);
+  return fma(c[4], qpp, acc);
 }
 
 template <typename T>
@@ -582,47 +612,32 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     if (have_plane && t >= kb && t < ke) {
       const T* U5 = sub;
       const int ro = r_own;
-      // x
+      // x, then y: the hybrid test (main_v1.9.f90:333-335) differs from node
+      // to node within a wave, so both weight sets are formed and the node's
+      // own is selected; one read of the stencil serves either scheme
       {
         const T* ul = sl[s][0];
-        if (T(ph.Re) * fabs(Uw[4][1]) * T(ph.dx) > T(ph.Pex) * mu_t) {
-          T qx[5][4];
-          const T um1 = ul[ro - 1], up1 = ul[ro + 1];
+        const bool quick = T(ph.Re) * fabs(Uw[4][1]) * T(ph.dx) > T(ph.Pex) * mu_t;
+        T qx[5][4];
+        const T um2 = ul[ro - 2], um1 = ul[ro - 1], up1 = ul[ro + 1], up2 = ul[ro + 2];
 #pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const T* q = U5 + c * NR + ro;
-            qx[c][0] = q[-2];
-            qx[c][1] = q[-1];
-            qx[c][2] = q[1];
-            qx[c][3] = q[2];
-          }
-          __builtin_amdgcn_sched_barrier(0);
-          const T t1 = (u_t + up1) * half, t2 = (u_t + um1) * half;
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            Sg[2][c] += quickT<T>(t1, t2, qx[c][0], qx[c][1], Uw[4][c], qx[c][2], qx[c][3]) *
-                        (invdx * T(0.125));
-        } else {
-          T qx[5][4];
-          const T um2 = ul[ro - 2], um1 = ul[ro - 1], up1 = ul[ro + 1], up2 = ul[ro + 2];
-#pragma unroll
-          for (int c = 0; c < 5; c++) {
-            const T* q = U5 + c * NR + ro;
-            qx[c][0] = q[-2];
-            qx[c][1] = q[-1];
-            qx[c][2] = q[1];
-            qx[c][3] = q[2];
-          }
-          __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-          for (int c = 0; c < 5; c++)
-            Sg[2][c] += (-up2 * qx[c][3] + T(8) * up1 * qx[c][2] - T(8) * um1 * qx[c][1] +
-                         um2 * qx[c][0]) * (invdx * T(kOne12th));
+        for (int c = 0; c < 5; c++) {
+          const T* q = U5 + c * NR + ro;
+          qx[c][0] = q[-2];
+          qx[c][1] = q[-1];
+          qx[c][2] = q[1];
+          qx[c][3] = q[2];
         }
+        __builtin_amdgcn_sched_barrier(0);
+        T wq[5], wc[5];
+        quick_w<T>((u_t + up1) * half, (u_t + um1) * half, invdx * T(0.125), wq);
+        central_w<T>(um2, um1, up1, up2, invdx * T(kOne12th), wc);
+#pragma unroll
+        for (int i = 0; i < 5; i++) wq[i] = quick ? wq[i] : wc[i];
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          Sg[2][c] = apply_w<T>(wq, qx[c][0], qx[c][1], Uw[4][c], qx[c][2], qx[c][3], Sg[2][c]);
       }
-      // y: the Peclet test differs from node to node within most waves, so
-      // both schemes are evaluated from one read of the stencil and the
-      // node's own is selected (a divergent branch would read it twice)
       {
         const T* vl = sl[s][1];
         const bool quick = T(ph.Re) * fabs(Uw[4][2]) * T(ph.dy) > T(ph.Pey) * mu_t;
@@ -637,15 +652,14 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
           qy[c][3] = q[2 * RX];
         }
         __builtin_amdgcn_sched_barrier(0);
-        const T t1 = (v_t + vp1) * half, t2 = (v_t + vm1) * half;
+        T wq[5], wc[5];
+        quick_w<T>((v_t + vp1) * half, (v_t + vm1) * half, invdy * T(0.125), wq);
+        central_w<T>(vm2, vm1, vp1, vp2, invdy * T(kOne12th), wc);
 #pragma unroll
-        for (int c = 0; c < 5; c++) {
-          const T fq = quickT<T>(t1, t2, qy[c][0], qy[c][1], Uw[4][c], qy[c][2], qy[c][3]) *
-                       (invdy * T(0.125));
-          const T fc = (-vp2 * qy[c][3] + T(8) * vp1 * qy[c][2] - T(8) * vm1 * qy[c][1] +
-                        vm2 * qy[c][0]) * (invdy * T(kOne12th));
-          Sg[2][c] += quick ? fq : fc;
-        }
+        for (int i = 0; i < 5; i++) wq[i] = quick ? wq[i] : wc[i];
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          Sg[2][c] = apply_w<T>(wq, qy[c][0], qy[c][1], Uw[4][c], qy[c][2], qy[c][3], Sg[2][c]);
       }
     }
     // no barrier: phase 3 reads plane t-1's slots and the z-ring (complete
@@ -833,21 +847,26 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         // window slots 0..4 (slot 2 = q)
         T H[5];
         const bool lowhigh = (q + 1 < 3) || (q + 1 > n3 - 2);
+        T wz[5];
         if (lowhigh) {
-  #pragma unroll
-          for (int c = 0; c < 5; c++) H[c] = (ww[3] * Uw[3][c] - ww[1] * Uw[1][c]) * (invdXi * iJ * half);
-        } else if (zq & 4u) {
-          const T t1 = (ww[2] + ww[3]) * half, t2 = (ww[2] + ww[1]) * half;
-  #pragma unroll
-          for (int c = 0; c < 5; c++)
-            H[c] = quickT<T>(t1, t2, Uw[0][c], Uw[1][c], Uw[2][c], Uw[3][c], Uw[4][c]) *
-                   (invdXi * iJ * T(0.125));
+          // second-order central next to the walls
+          const T k = invdXi * iJ * half;
+          wz[0] = T(0);
+          wz[1] = -ww[1] * k;
+          wz[2] = T(0);
+          wz[3] = ww[3] * k;
+          wz[4] = T(0);
         } else {
-  #pragma unroll
-          for (int c = 0; c < 5; c++)
-            H[c] = (-ww[4] * Uw[4][c] + T(8) * ww[3] * Uw[3][c] - T(8) * ww[1] * Uw[1][c] +
-                    ww[0] * Uw[0][c]) * (invdXi * iJ * T(kOne12th));
+          T wc[5];
+          quick_w<T>((ww[2] + ww[3]) * half, (ww[2] + ww[1]) * half, invdXi * iJ * T(0.125), wz);
+          central_w<T>(ww[0], ww[1], ww[3], ww[4], invdXi * iJ * T(kOne12th), wc);
+          const bool quick = (zq & 4u) != 0u;
+#pragma unroll
+          for (int i = 0; i < 5; i++) wz[i] = quick ? wz[i] : wc[i];
         }
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+          H[c] = apply_w<T>(wz, Uw[0][c], Uw[1][c], Uw[2][c], Uw[3][c], Uw[4][c], T(0));
         // (the z divergence of hD is already in Sg[0])
         double R[5];
         const double rb = rb_q;
