@@ -149,7 +149,7 @@ __global__ void k_consts(Geom g, Phys ph, int n1, int n2, const double* __restri
       }
   } else if (ph.Ls == 0.0 && !((wflags >> (kk != 0)) & 1)) {
     // (slip walls: the ring holds periodic images; study walls: the ring
-    // is updated every stage and summed by k_planestats)
+    // is updated every stage and summed by k_stats_part)
     for (int j = 0; j < n2; j++)
       for (int i = 0; i < n1; i++) {
         if (!(i == 0 || i == n1 - 1 || j == 0 || j == n2 - 1)) continue;
@@ -298,16 +298,13 @@ __device__ __forceinline__ void block_sum(double (&a)[NV], double* lds /* [nwave
 // (b, wall) sum a contiguous range of the wall's nodes, then one thread per
 // wall adds the kTauBlocks partials in order.
 constexpr int kTauBlocks = 32;
-__global__ void __launch_bounds__(256) k_tauwall_part(Geom g, Phys ph, const double* __restrict__ U,
-                                                      WallP pw, double* __restrict__ part,
-                                                      const int* halt, int step) {
-  __shared__ double lds[4];
-  const int h = *halt;
-  if (h != 0 && h <= step) return;
-  const bool top = blockIdx.y == 1;
+__device__ __forceinline__ void tauwall_part(const Geom& g, const Phys& ph,
+                                             const double* __restrict__ U, const WallP& pw,
+                                             double* __restrict__ part, int blk, bool top,
+                                             double* lds) {
   const long long F = g.field;
   const int N = g.n3 - 1;
-  const long long n0 = g.plane * blockIdx.x / kTauBlocks, n1e = g.plane * (blockIdx.x + 1) / kTauBlocks;
+  const long long n0 = g.plane * blk / kTauBlocks, n1e = g.plane * (blk + 1) / kTauBlocks;
   double acc[1] = {0.0};
   for (long long n = n0 + threadIdx.x; n < n1e; n += blockDim.x) {
     const int ii = (int)(n % g.P1), jj = (int)(n / g.P1);
@@ -325,39 +322,22 @@ __global__ void __launch_bounds__(256) k_tauwall_part(Geom g, Phys ph, const dou
     acc[0] += (dwdx + dudz) * ph.invRe;
   }
   block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) part[blockIdx.y * kTauBlocks + blockIdx.x] = acc[0];
-}
-
-__global__ void k_tauwall_final(int n1, int n2, const double* __restrict__ part,
-                                double* __restrict__ force, double* __restrict__ tau_out,
-                                const int* halt, int step) {
-  const int h = *halt;
-  if (h != 0 && h <= step) return;
-  const int w = threadIdx.x;
-  if (w >= 2) return;
-  double s = 0.0;
-  for (int b = 0; b < kTauBlocks; b++) s += part[w * kTauBlocks + b];
-  const double t = s / (double)((n1 - 2) * (n2 - 2));
-  force[w] = t;
-  tau_out[w] = t;
+  if (threadIdx.x == 0) part[(top ? kTauBlocks : 0) + blk] = acc[0];
 }
 
 // Plane statistics (main_v1.9.f90:993-1025) in two deterministic passes:
-// blocks (s, kk) sum one of kStatSplit node ranges of plane kk, then one
-// thread per plane adds the partials in order.  Each unique interior node is
+// blocks (s, kk) sum one of `split` node ranges of plane kk, then one thread
+// per plane adds the partials in order.  Each unique interior node is
 // weighted by the number of times the reference's all-node sum sees it (1 +
 // an x duplicate + a y duplicate); the stale corners and wall rings come in
 // through C, a study wall's duplicates through its ring.
-constexpr int kStatSplit = 8;
-__global__ void __launch_bounds__(256) k_planestats_part(Geom g, Phys ph,
-                                                         const double* __restrict__ U, WallP pw,
-                                                         const double* __restrict__ rw,
-                                                         double* __restrict__ part,
-                                                         const int* halt, int step) {
-  __shared__ double lds[4 * kNStat];
-  const int h = *halt;
-  if (h != 0 && h <= step) return;
-  const int kk = blockIdx.y, n3 = g.n3, sp = blockIdx.x;
+constexpr int kStatSplitMax = 8;
+__device__ __forceinline__ void planestats_part(const Geom& g, const Phys& ph,
+                                                const double* __restrict__ U, const WallP& pw,
+                                                const double* __restrict__ rw,
+                                                double* __restrict__ part, int kk, int sp,
+                                                int split, double* lds) {
+  const int n3 = g.n3;
   const bool wall = (kk == 0 || kk == n3 - 1);
   const long long F = g.field;
   double acc[kNStat];
@@ -365,7 +345,7 @@ __global__ void __launch_bounds__(256) k_planestats_part(Geom g, Phys ph,
   double tq[kNStat];
   // a plane holds fewer than 2^31 nodes (checked at create): 32-bit index
   // arithmetic
-  const int n0 = (int)(g.plane * sp / kStatSplit), n1e = (int)(g.plane * (sp + 1) / kStatSplit);
+  const int n0 = (int)(g.plane * sp / split), n1e = (int)(g.plane * (sp + 1) / split);
   for (int n = n0 + (int)threadIdx.x; n < n1e; n += blockDim.x) {
     const int jj = n / g.P1, ii = n - jj * g.P1;
     Prim P;
@@ -396,23 +376,54 @@ __global__ void __launch_bounds__(256) k_planestats_part(Geom g, Phys ph,
   }
   block_sum<kNStat>(acc, lds);
   if (threadIdx.x == 0)
-    for (int q = 0; q < kNStat; q++) part[((long long)q * n3 + kk) * kStatSplit + sp] = acc[q];
+    for (int q = 0; q < kNStat; q++) part[((long long)q * n3 + kk) * split + sp] = acc[q];
 }
 
-__global__ void k_planestats_final(int n1, int n2, int n3, const double* __restrict__ part,
-                                   const double* __restrict__ C, double* __restrict__ force,
-                                   double* __restrict__ stats, const int* halt, int step) {
+// First pass of the statistics step in one launch: blocks [0, split n3)
+// take the plane statistics (block b: plane b / split, range b % split),
+// the 2 kTauBlocks after them the wall shear (bottom wall first).
+__global__ void __launch_bounds__(256) k_stats_part(Geom g, Phys ph, const double* __restrict__ U,
+                                                    WallP pw, const double* __restrict__ rw,
+                                                    double* __restrict__ part, int split,
+                                                    const int* halt, int step) {
+  __shared__ double lds[4 * kNStat];
+  const int h = *halt;
+  if (h != 0 && h <= step) return;
+  const int b = blockIdx.x, nps = split * g.n3;
+  if (b < nps) {
+    planestats_part(g, ph, U, pw, rw, part, b / split, b % split, split, lds);
+  } else {
+    const int t = b - nps;
+    tauwall_part(g, ph, U, pw, part + (long long)kNStat * g.n3 * split, t % kTauBlocks,
+                 t >= kTauBlocks, lds);
+  }
+}
+
+// Second pass: thread kk < n3 adds plane kk's partials in order; threads
+// n3 and n3 + 1 the two walls' shear partials.
+__global__ void k_stats_final(int n1, int n2, int n3, const double* __restrict__ part, int split,
+                              const double* __restrict__ C, double* __restrict__ force,
+                              double* __restrict__ stats, const int* halt, int step) {
   const int h = *halt;
   if (h != 0 && h <= step) return;
   const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-  if (kk >= n3) return;
-  const double nXY = (double)(n1 * n2);
-  for (int q = 0; q < kNStat; q++) {
-    double a = 0.0;
-    for (int sp = 0; sp < kStatSplit; sp++) a += part[((long long)q * n3 + kk) * kStatSplit + sp];
-    const double t = (a + C[q * n3 + kk]) / nXY;
-    stats[q * n3 + kk] = t;
-    if (q == 0) force[2 + kk] = t;
+  if (kk < n3) {
+    const double nXY = (double)(n1 * n2);
+    for (int q = 0; q < kNStat; q++) {
+      double a = 0.0;
+      for (int sp = 0; sp < split; sp++) a += part[((long long)q * n3 + kk) * split + sp];
+      const double t = (a + C[q * n3 + kk]) / nXY;
+      stats[q * n3 + kk] = t;
+      if (q == 0) force[2 + kk] = t;
+    }
+  } else if (kk < n3 + 2) {
+    const int w = kk - n3;
+    const double* tp = part + (long long)kNStat * n3 * split;
+    double s = 0.0;
+    for (int b = 0; b < kTauBlocks; b++) s += tp[w * kTauBlocks + b];
+    const double t = s / (double)((n1 - 2) * (n2 - 2));
+    force[w] = t;
+    stats[kNStat * n3 + w] = t;
   }
 }
 
@@ -603,22 +614,40 @@ void launch_download(const Geom& g, const Phys& ph, int n1, int n2, const double
                      n2, U, pw, rw, Vfull, Vout);
 }
 
-int stats_scratch_size(int n3) { return kNStat * n3 * kStatSplit + 2 * kTauBlocks; }
+int stats_scratch_size(int n3) { return kNStat * n3 * kStatSplitMax + 2 * kTauBlocks; }
+
+// Node ranges per plane of the statistics pass: as many as let the whole
+// first pass (split n3 + 2 kTauBlocks blocks) run in one round on the
+// current device, at most kStatSplitMax (a second, partly filled round of
+// blocks cost more than the longer ranges).
+static int stats_split(int n3) {
+  static int resident[kMaxDevices] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  if (resident[dev] == 0) {
+    int ncu = 0, bpc = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+      ncu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_stats_part, 256, 0) != hipSuccess ||
+        bpc <= 0)
+      bpc = 1;
+    resident[dev] = ncu * bpc;
+  }
+  const int sp = (resident[dev] - 2 * kTauBlocks) / n3;
+  return sp < 1 ? 1 : (sp > kStatSplitMax ? kStatSplitMax : sp);
+}
 
 void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
                   const double* rw, const double* consts, double* force, double* stats,
                   double* part, const int* halt, int step, hipStream_t s) {
   // stats layout: [23][n3] plane statistics, then tau[2]; part: the plane
-  // partials [23][n3][kStatSplit], then the wall-shear partials [2][kTauBlocks]
-  double* tpart = part + kNStat * g.n3 * kStatSplit;
-  hipLaunchKernelGGL(k_tauwall_part, dim3(kTauBlocks, 2), dim3(256), 0, s, g, ph, U, pw, tpart,
-                     halt, step);
-  hipLaunchKernelGGL(k_tauwall_final, dim3(1), dim3(64), 0, s, n1, n2, tpart, force,
-                     stats + kNStat * g.n3, halt, step);
-  hipLaunchKernelGGL(k_planestats_part, dim3(kStatSplit, g.n3), dim3(256), 0, s, g, ph, U, pw, rw,
-                     part, halt, step);
-  hipLaunchKernelGGL(k_planestats_final, dim3((g.n3 + 63) / 64), dim3(64), 0, s, n1, n2, g.n3,
-                     part, consts, force, stats, halt, step);
+  // partials [23][n3][split], then the wall-shear partials [2][kTauBlocks]
+  const int split = stats_split(g.n3);
+  hipLaunchKernelGGL(k_stats_part, dim3((unsigned)(split * g.n3 + 2 * kTauBlocks)), dim3(256), 0,
+                     s, g, ph, U, pw, rw, part, split, halt, step);
+  hipLaunchKernelGGL(k_stats_final, dim3((g.n3 + 2 + 63) / 64), dim3(64), 0, s, n1, n2, g.n3,
+                     part, split, consts, force, stats, halt, step);
 }
 
 }  // namespace cc
