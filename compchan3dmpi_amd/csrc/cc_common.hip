@@ -346,24 +346,32 @@ __device__ __forceinline__ void planestats_part(const Geom& g, const Phys& ph,
   // a plane holds fewer than 2^31 nodes (checked at create): 32-bit index
   // arithmetic
   const int n0 = (int)(g.plane * sp / split), n1e = (int)(g.plane * (sp + 1) / split);
-  for (int n = n0 + (int)threadIdx.x; n < n1e; n += blockDim.x) {
-    const int jj = n / g.P1, ii = n - jj * g.P1;
-    Prim P;
-    double wgt = 1.0;
-    if (wall) {
+  const int bd = (int)blockDim.x;
+  if (wall) {
+    for (int n = n0 + (int)threadIdx.x; n < n1e; n += bd) {
+      const int jj = n / g.P1, ii = n - jj * g.P1;
+      double wgt = 1.0;
       const bool top = kk == n3 - 1;
       const long long wo = (top ? g.plane : 0) + n;
-      P = wall_prim(ph, rw[wo], pw.p[(top ? pw.top_off : 0) + n], pw.uw ? pw.uw[wo] : 0.0,
-                    pw.vw ? pw.vw[wo] : 0.0, pw.tw ? pw.tw[wo] : 1.0);
+      const Prim P = wall_prim(ph, rw[wo], pw.p[(top ? pw.top_off : 0) + n],
+                               pw.uw ? pw.uw[wo] : 0.0, pw.vw ? pw.vw[wo] : 0.0,
+                               pw.tw ? pw.tw[wo] : 1.0);
       if (ph.Ls != 0.0 && !((pw.wflags >> top) & 1))  // periodic-image ring: every duplicate counts
         wgt = ((ii == 0 || ii == g.P1 - 1) ? 2.0 : 1.0) * ((jj == 0 || jj == g.P2 - 1) ? 2.0 : 1.0);
-    } else {
-      const long long m = idx(g, ii, jj, kk);
-      P = prim_of(ph, U[m], U[F + m], U[2 * F + m], U[3 * F + m], U[4 * F + m]);
-      wgt = 1.0 + ((ii == 0 || ii == g.P1 - 1) ? 1.0 : 0.0) + ((jj == 0 || jj == g.P2 - 1) ? 1.0 : 0.0);
+      stat_terms(P, tq);
+      for (int q = 0; q < kNStat; q++) acc[q] += wgt * tq[q];
     }
-    stat_terms(P, tq);
-    for (int q = 0; q < kNStat; q++) acc[q] += wgt * tq[q];
+  } else {
+    const long long base = (long long)kk * g.plane;
+    for (int n = n0 + (int)threadIdx.x; n < n1e; n += bd) {
+      const int jj = n / g.P1, ii = n - jj * g.P1;
+      const long long m = base + n;
+      const Prim P = prim_of(ph, U[m], U[F + m], U[2 * F + m], U[3 * F + m], U[4 * F + m]);
+      const double wgt =
+          1.0 + ((ii == 0 || ii == g.P1 - 1) ? 1.0 : 0.0) + ((jj == 0 || jj == g.P2 - 1) ? 1.0 : 0.0);
+      stat_terms(P, tq);
+      for (int q = 0; q < kNStat; q++) acc[q] += wgt * tq[q];
+    }
   }
   if (wall && sp == 0 && ((pw.wflags >> (kk != 0)) & 1)) {
     // study wall: the duplicates as the last wall update left them
@@ -399,25 +407,30 @@ __global__ void __launch_bounds__(256) k_stats_part(Geom g, Phys ph, const doubl
   }
 }
 
-// Second pass: thread kk < n3 adds plane kk's partials in order; threads
-// n3 and n3 + 1 the two walls' shear partials.
+// Second pass: thread e < kNStat n3 adds the partials of quantity q = e / n3
+// on plane kk = e % n3 in order (all of them loaded first: a loop of
+// dependent loads made this pass latency-bound); threads kNStat n3 and
+// kNStat n3 + 1 the two walls' shear partials.
 __global__ void k_stats_final(int n1, int n2, int n3, const double* __restrict__ part, int split,
                               const double* __restrict__ C, double* __restrict__ force,
                               double* __restrict__ stats, const int* halt, int step) {
   const int h = *halt;
   if (h != 0 && h <= step) return;
-  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-  if (kk < n3) {
-    const double nXY = (double)(n1 * n2);
-    for (int q = 0; q < kNStat; q++) {
-      double a = 0.0;
-      for (int sp = 0; sp < split; sp++) a += part[((long long)q * n3 + kk) * split + sp];
-      const double t = (a + C[q * n3 + kk]) / nXY;
-      stats[q * n3 + kk] = t;
-      if (q == 0) force[2 + kk] = t;
-    }
-  } else if (kk < n3 + 2) {
-    const int w = kk - n3;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < kNStat * n3) {
+    const double* pp = part + (long long)e * split;  // [q][kk][split] with e = q n3 + kk
+    double v[kStatSplitMax];
+#pragma unroll
+    for (int sp = 0; sp < kStatSplitMax; sp++) v[sp] = sp < split ? pp[sp] : 0.0;
+    double a = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < kStatSplitMax; sp++)
+      if (sp < split) a += v[sp];
+    const double t = (a + C[e]) / (double)(n1 * n2);
+    stats[e] = t;
+    if (e < n3) force[2 + e] = t;  // q = 0: the mean density of plane e
+  } else if (e < kNStat * n3 + 2) {
+    const int w = e - kNStat * n3;
     const double* tp = part + (long long)kNStat * n3 * split;
     double s = 0.0;
     for (int b = 0; b < kTauBlocks; b++) s += tp[w * kTauBlocks + b];
@@ -646,8 +659,8 @@ void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U
   const int split = stats_split(g.n3);
   hipLaunchKernelGGL(k_stats_part, dim3((unsigned)(split * g.n3 + 2 * kTauBlocks)), dim3(256), 0,
                      s, g, ph, U, pw, rw, part, split, halt, step);
-  hipLaunchKernelGGL(k_stats_final, dim3((g.n3 + 2 + 63) / 64), dim3(64), 0, s, n1, n2, g.n3,
-                     part, split, consts, force, stats, halt, step);
+  hipLaunchKernelGGL(k_stats_final, dim3((kNStat * g.n3 + 2 + 255) / 256), dim3(256), 0, s, n1,
+                     n2, g.n3, part, split, consts, force, stats, halt, step);
 }
 
 }  // namespace cc
