@@ -327,7 +327,7 @@ __device__ __forceinline__ void tauwall_part(const Geom& g, const Phys& ph,
 
 // Plane statistics (main_v1.9.f90:993-1025) in two deterministic passes:
 // blocks (s, kk) sum one of `split` node ranges of plane kk, then one thread
-// per plane adds the partials in order.  Each unique interior node is
+// per quantity and plane adds the partials in order.  Each unique interior node is
 // weighted by the number of times the reference's all-node sum sees it (1 +
 // an x duplicate + a y duplicate); the stale corners and wall rings come in
 // through C, a study wall's duplicates through its ring.
