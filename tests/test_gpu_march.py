@@ -168,6 +168,40 @@ def test_c2_fp64_and_strict_match_the_oracle(c2_start):
         del vg
 
 
+@pytest.mark.timeout(600)
+def test_c2_benchmark_step_100_steps_matches_the_oracle(c2_start):
+    """The benchmark's own configuration (nP=[256,128,192], dt = 9e-5,
+    istat = 10, the production FP64 tiling) for 100 steps, as config 1 is
+    run: ten statistics steps, the forcing on from step 11, against the
+    oracle."""
+    v0 = c2_start
+    banner("c2 256x128x192 oracle, 100 steps istat=10")
+    o = orc.Oracle(*C2, dt=DT2, istat=10)
+    o.load(v0)
+    assert o.step(100) == 0
+    vo = o.var()
+    tau_o, s1o, _, _ = o.stats()
+    Co = conserved(vo)
+    del o
+    banner("c2 256x128x192 100 steps istat=10 fp64")
+    ch = Channel(*C2, dt=DT2, istat=10, precision="fp64")
+    try:
+        ch.upload(v0)
+        assert ch.step(100) == 0
+        vg, ns, _ = ch.download()
+        tau_g, s1g, _, _ = ch.stats()
+    finally:
+        ch.close()
+    assert ns == 100
+    e = relL2(conserved(vg), Co)
+    print(f"  rel L2 conserved = {e:.3e}", flush=True)
+    assert e <= 1e-12
+    for c in range(7):
+        assert relL2(vg[c], vo[c]) <= 1e-12, NAMES[c]
+    assert np.allclose(tau_g, tau_o, rtol=1e-11, atol=1e-15)
+    assert np.allclose(s1g, s1o, rtol=1e-12, atol=1e-15)
+
+
 @pytest.mark.timeout(300)
 def test_c3_mixed_against_fp64_after_1000_steps(c2_start):
     """Config 3: per-field relative L2 error of the mixed mode against the
