@@ -997,7 +997,7 @@ int cc_batch_tune(void* batch, int trial_steps, double* out) {
   int step0 = 0;
   for (Chan* c : b->m) step0 = std::max(step0, c->nstep);
   std::vector<int> variants = b->mixed ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0};
-  const int kcs[] = {0, 8, 16, 32, 64};
+  const int kcs[] = {0, 8, 12, 16, 24, 32, 48, 64};
   hipEvent_t e0, e1;
   // timing only (see cc_step): no system-scope fence between the trials
   HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
