@@ -15,7 +15,8 @@ periodic duplicates and wall planes included; config.cells also gives the
 figure is reported in config.mixed, with its own roofline and the per-field
 error against FP64 after 1000 steps (config 3), whatever --steps is.
 roofline: achieved algorithmic HBM bandwidth of the stage kernel over its
-average launch time measured with HIP events on the library's stream.  FP64
+average launch time measured with HIP events on the library's stream, around
+the stage launches of every tenth timed step (every step of a short run).  FP64
 moves 100 B per node and stage (stage 1 reads U^n and writes U*, stage 2
 reads U*, U^n and writes U^{n+1}; 5 FP64 words each); mixed keeps U* in
 FP32, 80 B.  cpu_baseline: the reference itself (oracle/_ref) on one MPI rank
@@ -52,6 +53,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profile_every(steps):
+    """Stage launches timed by HIP events: those of every tenth step of a
+    long run (the events of the other steps would only add their own cost to
+    the throughput), every step of a short one."""
+    return 10 if steps >= 200 else 1
+
+
 def run_gpu(var, precision, steps, warmup, profile, keep=False):
     n1, n2, n3 = NP
     ch = Channel(n1, n2, n3, dt=DT, istat=ISTAT, precision=precision)
@@ -60,7 +68,7 @@ def run_gpu(var, precision, steps, warmup, profile, keep=False):
     if rc:
         raise RuntimeError(f"{precision}: blow-up at warm-up step {rc}")
     if profile:
-        ch.profile(1)
+        ch.profile(-profile_every(steps))
     t0 = time.perf_counter()
     rc = ch.step(steps)          # returns after a device synchronisation
     t1 = time.perf_counter()
@@ -259,7 +267,8 @@ def main():
                               "traffic": traffic_from_profiles("mixed"),
                               "alg_bytes_per_node_stage": ALG_BYTES_PER_NODE_STAGE_MIXED,
                               "kernel": "k_stage_fused<float>",
-                              "avg_launch_ms": 1000 * mix_launch_s},
+                              "avg_launch_ms": 1000 * mix_launch_s,
+                              "launches_timed": knm},
                  "error_steps": ERR_STEPS,
                  "rel_l2_vs_fp64": err,
                  "mean_u_profile_rel_err": float(np.abs(ubm - ub64).max() / np.abs(ub64).max()),
@@ -319,6 +328,7 @@ def main():
             "kernel": "k_stage_fused" if args.precision != "fp64-strict" else "k_rhs_strict",
             "alg_bytes_per_node_stage": ALG_BYTES_PER_NODE_STAGE,
             "avg_launch_ms": 1000 * avg_launch_s,
+            "launches_timed": kn,
         },
         "cpu_baseline": cpu,
     }

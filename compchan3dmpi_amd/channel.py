@@ -236,7 +236,8 @@ class Channel:
 
     def profile(self, mode=2):
         """mode 1: reset and enable per-launch HIP-event timing of the stage
-        kernel; 0: disable; 2: query.  Returns (total_ms, launches)."""
+        kernel; -k (k >= 2): the same for every k-th step only; 0: disable;
+        2: query.  Returns (total_ms, launches)."""
         out = np.zeros(2)
         _check(self.L, self.L.cc_profile(self.h, mode, out))
         return float(out[0]), int(out[1])

@@ -70,6 +70,7 @@ struct Chan {
   int tile_ncu = 0, tile_bpc = 0;  // CUs and resident blocks per CU the tiling assumed
   // per-launch timing of the stage's main kernel (cc_profile)
   bool prof = false;
+  int prof_every = 1;  // time the stage launches of every prof_every-th step
   std::vector<hipEvent_t> evpool;
   double prof_ms = 0.0;
   long long prof_n = 0;
@@ -388,7 +389,12 @@ int cc_step(void* handle, int nsteps) {
       c->evpool.push_back(e);
     }
   }
-  auto ev = [&](int s, int k) -> hipEvent_t { return c->prof ? c->evpool[(size_t)4 * s + k] : nullptr; };
+  // the steps whose stage launches are timed: every prof_every-th step by
+  // its number, so that a sample spreads evenly over the run
+  auto timed = [&](int s) { return c->prof && (step0 + 1 + s) % c->prof_every == 0; };
+  auto ev = [&](int s, int k) -> hipEvent_t {
+    return timed(s) ? c->evpool[(size_t)4 * s + k] : nullptr;
+  };
   for (int s = 0; s < nsteps; s++) {
     const int step = c->nstep + 1;
     c->inst.time = c->time + c->prm.dt;  // the reference advances time first
@@ -429,7 +435,7 @@ int cc_step(void* handle, int nsteps) {
   if (nsteps > 0) c->fresh = false;
   if (c->prof)
     for (int s = 0; s < nsteps; s++)
-      for (int k = 0; k < 2; k++) {
+      for (int k = 0; k < 2 && timed(s); k++) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ev(s, 2 * k), ev(s, 2 * k + 1)));
         c->prof_ms += ms;
@@ -500,8 +506,9 @@ int cc_blown(void* handle, int* ijk) {
 int cc_profile(void* handle, int mode, double* out) {
   Chan* c = (Chan*)handle;
   if (!c) return fail(CC_EINVAL, "null handle");
-  if (mode == 1) {
+  if (mode == 1 || mode < 0) {
     c->prof = true;
+    c->prof_every = mode < 0 ? (mode < -1000000 ? 1000000 : -mode) : 1;
     c->prof_ms = 0.0;
     c->prof_n = 0;
   } else if (mode == 0) {

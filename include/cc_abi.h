@@ -104,9 +104,12 @@ int cc_blown(void* handle, int* ijk);
 
 /* Per-launch timing of each RK stage's main kernel (no counterpart in the
  * reference, which times 10 steps with cpu_time at main_v1.9.f90:322,1042):
- * mode 1 resets and enables HIP-event timing on the handle's stream, mode 0
- * disables it, any other mode only queries.  out[0] = summed kernel time in
- * ms, out[1] = number of timed launches since the reset (out may be NULL). */
+ * mode 1 resets and enables HIP-event timing on the handle's stream for the
+ * stage launches of every step; mode -k (k >= 2) does the same for the steps
+ * whose number is a multiple of k (a sample that leaves the other steps
+ * without timing events); mode 0 disables it; any other mode only queries.
+ * out[0] = summed kernel time in ms, out[1] = number of timed launches since
+ * the reset (out may be NULL). */
 int cc_profile(void* handle, int mode, double* out);
 
 /* Tiling the fused stage kernel uses on this handle (no counterpart in the

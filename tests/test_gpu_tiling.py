@@ -54,3 +54,36 @@ def test_tiling_is_the_model_optimum(grid, precision):
     assert all(cost(k) > best for k in range(kc + 1, nint + 1))
     if t["cus"] == 256 and (grid, precision) in EXPECTED_256:
         assert ((tx, ty), kc, t["blocks"]) == EXPECTED_256[(grid, precision)], t
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64-strict", "mixed"])
+def test_profile_sampling_counts_and_leaves_results_alone(precision):
+    """cc_profile: mode 1 times both stage launches of every step, mode -k
+    those of the steps whose number is a multiple of k; the timing events
+    change no result bit."""
+    import numpy as np
+
+    from compchan3dmpi_amd import formats
+
+    n = (24, 20, 16)
+    v0 = formats.laminar_ic(*n)
+    out = []
+    for mode in (0, 1, -5):
+        ch = Channel(*n, dt=1e-3, istat=5, precision=precision)
+        try:
+            ch.upload(v0)
+            assert ch.step(3) == 0
+            if mode:
+                ch.profile(mode)
+            assert ch.step(20) == 0  # steps 4..23
+            ms, launches = ch.profile(0)
+            out.append(ch.download()[0])
+        finally:
+            ch.close()
+        if mode == 1:
+            assert launches == 40 and ms > 0.0
+        elif mode == -5:
+            assert launches == 2 * 4 and ms > 0.0  # steps 5, 10, 15, 20
+        else:
+            assert launches == 0
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
