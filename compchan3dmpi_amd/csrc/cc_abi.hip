@@ -181,6 +181,13 @@ struct Batch {
   unsigned base = 0;
   int variant = 0, kchunk = 0, nblocks = 0, ncu = 0;
   int maxP1 = 0, maxP2 = 0;
+  // the members' statistics in one pair of launches (every member on the
+  // same istat): their descriptors, host copy and device copy, and the
+  // block counts of the two passes
+  std::vector<StatDesc> sdesc;
+  StatDesc* d_sdesc = nullptr;
+  int s_part = 0, s_final = 0;
+  bool s_batched = false;
 };
 
 // Work list for the current tile shape: every (tile, z-chunk) of every
@@ -230,13 +237,48 @@ void batch_enqueue_step(Batch* b, int step, bool stats, hipEvent_t e0, hipEvent_
                          st == 1 ? e0 : e2, st == 1 ? e1 : e3);
     b->base += (unsigned)(b->nwork + b->nblocks);
   }
-  if (stats)
-    for (Chan* c : b->m)
-      if (step % c->prm.istat == 0) {
-        launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
-                     c->stats, c->spart, c->halt, step, b->s);
+  if (!stats) return;
+  if (b->s_batched) {
+    if (step % b->m[0]->prm.istat == 0) {
+      launch_stats_batched(b->d_sdesc, n, b->s_part, b->s_final, step, b->s);
+      for (Chan* c : b->m)
         if (c->averaging) launch_avg_accum(c->n3, c->stats, c->avgsum, c->halt, step, b->s);
-      }
+    }
+    return;
+  }
+  for (Chan* c : b->m)
+    if (step % c->prm.istat == 0) {
+      launch_stats(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
+                   c->stats, c->spart, c->halt, step, b->s);
+      if (c->averaging) launch_avg_accum(c->n3, c->stats, c->avgsum, c->halt, step, b->s);
+    }
+}
+
+// Descriptors of the batched statistics (rebuilt per cc_batch_step: a wall
+// condition set on a member changes its WallP); the per-member launches
+// serve members on different istat.
+int build_stats_desc(Batch* b) {
+  b->s_batched = true;
+  for (Chan* c : b->m)
+    if (c->prm.istat != b->m[0]->prm.istat) b->s_batched = false;
+  if (!b->s_batched) return 0;
+  b->sdesc.resize(b->m.size());
+  int pb = 0, fb = 0;
+  for (size_t i = 0; i < b->m.size(); i++) {
+    Chan* c = b->m[i];
+    StatDesc& d = b->sdesc[i];
+    d = stats_desc(c->g, c->ph, c->n1, c->n2, c->UA, c->wallp(), c->rw, c->consts, c->force,
+                   c->stats, c->spart, c->halt);
+    d.blk0 = pb;
+    d.fblk0 = fb;
+    pb += stats_part_blocks(d);
+    fb += stats_final_blocks(d);
+  }
+  b->s_part = pb;
+  b->s_final = fb;
+  HIPCHK(hipMemcpyAsync(b->d_sdesc, b->sdesc.data(), b->sdesc.size() * sizeof(StatDesc),
+                        hipMemcpyHostToDevice, b->s));
+  return 0;
 }
 
 }  // namespace
@@ -906,6 +948,7 @@ int cc_batch_create(const cc_params* p, int n, void** batch) {
     b->ncu = 256;
   if (hipStreamCreate(&b->s) != hipSuccess ||
       hipMalloc((void**)&b->d_insts, n * sizeof(Inst)) != hipSuccess ||
+      hipMalloc((void**)&b->d_sdesc, n * sizeof(StatDesc)) != hipSuccess ||
       hipMalloc((void**)&b->counter, sizeof(unsigned)) != hipSuccess ||
       hipMemset(b->counter, 0, sizeof(unsigned)) != hipSuccess) {
     cc_batch_destroy(b);
@@ -940,6 +983,7 @@ int cc_batch_step(void* batch, int nsteps) {
     }
   }
   if (step0 < 0) return fail(CC_ESTATE, "every batch member has stopped");
+  if (const int rc = build_stats_desc(b)) return rc;
   for (int s = 1; s <= nsteps; s++) batch_enqueue_step(b, step0 + s, true, nullptr, nullptr, nullptr, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->s));
@@ -1069,6 +1113,7 @@ void cc_batch_destroy(void* batch) {
   if (!b) return;
   for (Chan* c : b->m) free_chan(c);
   if (b->d_insts) (void)hipFree(b->d_insts);
+  if (b->d_sdesc) (void)hipFree(b->d_sdesc);
   if (b->d_work) (void)hipFree(b->d_work);
   if (b->counter) (void)hipFree(b->counter);
   if (b->s) (void)hipStreamDestroy(b->s);

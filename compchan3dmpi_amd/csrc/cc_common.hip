@@ -411,12 +411,11 @@ __global__ void __launch_bounds__(256) k_stats_part(Geom g, Phys ph, const doubl
 // on plane kk = e % n3 in order (all of them loaded first: a loop of
 // dependent loads made this pass latency-bound); threads kNStat n3 and
 // kNStat n3 + 1 the two walls' shear partials.
-__global__ void k_stats_final(int n1, int n2, int n3, const double* __restrict__ part, int split,
-                              const double* __restrict__ C, double* __restrict__ force,
-                              double* __restrict__ stats, const int* halt, int step) {
-  const int h = *halt;
-  if (h != 0 && h <= step) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void stats_final_elem(int e, int n1, int n2, int n3,
+                                                 const double* __restrict__ part, int split,
+                                                 const double* __restrict__ C,
+                                                 double* __restrict__ force,
+                                                 double* __restrict__ stats) {
   if (e < kNStat * n3) {
     const double* pp = part + (long long)e * split;  // [q][kk][split] with e = q n3 + kk
     double v[kStatSplitMax];
@@ -438,6 +437,52 @@ __global__ void k_stats_final(int n1, int n2, int n3, const double* __restrict__
     force[w] = t;
     stats[kNStat * n3 + w] = t;
   }
+}
+
+__global__ void k_stats_final(int n1, int n2, int n3, const double* __restrict__ part, int split,
+                              const double* __restrict__ C, double* __restrict__ force,
+                              double* __restrict__ stats, const int* halt, int step) {
+  const int h = *halt;
+  if (h != 0 && h <= step) return;
+  stats_final_elem(blockIdx.x * blockDim.x + threadIdx.x, n1, n2, n3, part, split, C, force,
+                   stats);
+}
+
+// The same two passes for the members of a batch, in one launch each: a
+// block belongs to the last member whose first block it has reached (the
+// members' blocks are contiguous), and runs that member's block of the
+// single-instance pass, so every member's sums are the single launch's.
+__device__ __forceinline__ int stats_member(const StatDesc* __restrict__ d, int nm, bool fin) {
+  const int b = (int)blockIdx.x;
+  int m = 0;
+  for (int i = 1; i < nm; i++)
+    if ((fin ? d[i].fblk0 : d[i].blk0) <= b) m = i;
+  return __builtin_amdgcn_readfirstlane(m);
+}
+
+__global__ void __launch_bounds__(256) k_stats_part_batched(const StatDesc* __restrict__ d,
+                                                            int nm, int step) {
+  __shared__ double lds[4 * kNStat];
+  const StatDesc& D = d[stats_member(d, nm, false)];
+  const int h = *D.halt;
+  if (h != 0 && h <= step) return;
+  const int b = (int)blockIdx.x - D.blk0, split = D.split, nps = split * D.g.n3;
+  if (b < nps) {
+    planestats_part(D.g, D.ph, D.U, D.pw, D.rw, D.part, b / split, b % split, split, lds);
+  } else {
+    const int t = b - nps;
+    tauwall_part(D.g, D.ph, D.U, D.pw, D.part + (long long)kNStat * D.g.n3 * split,
+                 t % kTauBlocks, t >= kTauBlocks, lds);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_stats_final_batched(const StatDesc* __restrict__ d,
+                                                             int nm, int step) {
+  const StatDesc& D = d[stats_member(d, nm, true)];
+  const int h = *D.halt;
+  if (h != 0 && h <= step) return;
+  stats_final_elem(((int)blockIdx.x - D.fblk0) * 256 + (int)threadIdx.x, D.n1, D.n2, D.g.n3,
+                   D.part, D.split, D.C, D.force, D.stats);
 }
 
 // ---- bulk diagnostics (main_v1.9.f90:799-906) --------------------------------
@@ -649,6 +694,36 @@ static int stats_split(int n3) {
   }
   const int sp = (resident[dev] - 2 * kTauBlocks) / n3;
   return sp < 1 ? 1 : (sp > kStatSplitMax ? kStatSplitMax : sp);
+}
+
+StatDesc stats_desc(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
+                    const double* rw, const double* consts, double* force, double* stats,
+                    double* part, const int* halt) {
+  StatDesc d{};
+  d.g = g;
+  d.ph = ph;
+  d.n1 = n1;
+  d.n2 = n2;
+  d.split = stats_split(g.n3);
+  d.U = U;
+  d.pw = pw;
+  d.rw = rw;
+  d.C = consts;
+  d.force = force;
+  d.stats = stats;
+  d.part = part;
+  d.halt = halt;
+  return d;
+}
+int stats_part_blocks(const StatDesc& d) { return d.split * d.g.n3 + 2 * kTauBlocks; }
+int stats_final_blocks(const StatDesc& d) { return (kNStat * d.g.n3 + 2 + 255) / 256; }
+
+void launch_stats_batched(const StatDesc* d_desc, int nm, int part_blocks, int final_blocks,
+                          int step, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_part_batched, dim3((unsigned)part_blocks), dim3(256), 0, s, d_desc,
+                     nm, step);
+  hipLaunchKernelGGL(k_stats_final_batched, dim3((unsigned)final_blocks), dim3(256), 0, s,
+                     d_desc, nm, step);
 }
 
 void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,

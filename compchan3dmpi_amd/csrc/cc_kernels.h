@@ -147,6 +147,36 @@ void launch_stats(const Geom& g, const Phys& ph, int n1, int n2, const double* U
                   const double* rw, const double* consts, double* force, double* stats,
                   double* part, const int* halt, int step, hipStream_t s);
 int stats_scratch_size(int n3);
+// One member of a batched statistics step: a single instance's launch_stats
+// arguments, its node ranges per plane, and where its blocks start in the
+// two batched passes (blk0: first pass, fblk0: second pass).
+struct StatDesc {
+  Geom g;
+  Phys ph;
+  int n1, n2, split;
+  int blk0, fblk0;
+  const double* U;
+  WallP pw;
+  const double* rw;
+  const double* C;
+  double* force;
+  double* stats;
+  double* part;
+  const int* halt;
+};
+// The descriptor of one instance, with the node ranges launch_stats uses
+// (blk0 and fblk0 are left at 0 for the caller), and its block counts in the
+// two passes.
+StatDesc stats_desc(const Geom& g, const Phys& ph, int n1, int n2, const double* U, WallP pw,
+                    const double* rw, const double* consts, double* force, double* stats,
+                    double* part, const int* halt);
+int stats_part_blocks(const StatDesc& d);
+int stats_final_blocks(const StatDesc& d);
+// The statistics step of nm members in two launches in all (d_desc on the
+// device, blk0/fblk0 the running block counts); each member's results are
+// bit-identical to its own launch_stats.
+void launch_stats_batched(const StatDesc* d_desc, int nm, int part_blocks, int final_blocks,
+                          int step, hipStream_t s);
 
 // ---- cc_post.hip: post-processing ------------------------------------------
 // sums[23*n3] += stats, sums[23*n3] += 1 (skipped on a halted run)

@@ -61,9 +61,13 @@ def run_single(sp, var, steps):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_members_equal_single_runs(precision):
-    banner(f"batch of 3 uneven instances vs single runs, {precision}")
-    specs = [dict(sp, precision=precision) for sp in SPECS]
+@pytest.mark.parametrize("istats", [(4, 4, 4), (4, 5, 4)])
+def test_batch_members_equal_single_runs(precision, istats):
+    """Members on one istat take their statistics in one batched pair of
+    launches, members on different istats one pair each; either way every
+    member's state and statistics equal its single run's, bit for bit."""
+    banner(f"batch of 3 uneven instances vs single runs, {precision}, istat {istats}")
+    specs = [dict(sp, precision=precision, istat=k) for sp, k in zip(SPECS, istats)]
     vars0 = [ic(sp["n1"], sp["n2"], sp["n3"], seed=7 + i) for i, sp in enumerate(specs)]
     b = Batch(specs)
     for m, v in zip(b.members, vars0):
