@@ -33,10 +33,12 @@
 // vector-load wait drains the state loads in flight.  FP32 prefetches the
 // next plane into registers after phase 3 and keeps its z window in FP32,
 // which holds it to 168 VGPRs and three waves per SIMD; in stage 2 it loads
-// the base state U^n of phase 4 after the phase-1 barrier.  FP64, out of
-// registers, warms the next plane in L2 with unread LDS-DMA loads, one per
-// cache line, during phases 2-3, and in stage 2 warms U^n the same way and
-// loads it after the phase-3 barrier, so that phase 3b hides the L2 hit.
+// the base state U^n of phase 4 after the phase-1 barrier.  FP64 stage 1
+// loads the next plane into registers just before phase 4, which needs no
+// base state from memory.  FP64 stage 2, out of registers, warms the next
+// plane in L2 with unread LDS-DMA loads, one per cache line, during phases
+// 2-3, warms U^n the same way and loads it after the phase-3 barrier, so
+// that phase 3b hides the L2 hit.  The batched kernel warms in both stages.
 #include <type_traits>
 
 #include "cc_device.h"
@@ -439,9 +441,13 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   T u_t = T(0), v_t = T(0), mu_t = T(0);  // u, v, mu of the own node of plane t (phase 1 -> 2)
   const int tb = max(kb - 2, 0), tend = ke + 1;  // inclusive
   const int tlast = min(n3 - 1, tend);             // last plane loaded
-  // The mixed kernel has registers to spare (FP64 runs at 256): right after
-  // the barrier that ends phase 3 it loads the next plane's state, so that
-  // phases 3b and 4 hide its latency.  FP64 loads it where it is used.
+  // The next plane's state is loaded a phase ahead into registers, so that
+  // its latency is hidden: the mixed kernel, which has registers to spare,
+  // right after the barrier that ends phase 3 (phases 3b and 4 hide it);
+  // the one-instance FP64 stage-argument kernel (stage 1 in practice) at
+  // the start of phase 4, whose register pressure is below phase 3's (the
+  // FP64 stage-2 and batched kernels would spill: they load it in phase 1,
+  // from L2, where the warming loads of phases 2-3 have put it).
   // Stage 2 reads its base state U^n into Ub: the mixed kernel after the
   // phase-1 barrier, FP64 after the phase-3 barrier (from L2, warmed after
   // the phase-1 barrier; held longer, it would not fit in 256 VGPRs).
@@ -449,7 +455,8 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   // whose window is FP32, rounded to FP32, which the FP32 predictor it
   // produces absorbs.  The FP64 stage-argument kernel (stage 1 only, in
   // practice) loads it in phase 4 when run for stage 2.
-  constexpr bool kAhead = sizeof(T) == 4;
+  constexpr bool kAhead = sizeof(T) == 4 || (S != 2 && std::is_same<I, Inst>::value);
+  constexpr bool kUbEarly = sizeof(T) == 4;
   constexpr bool kUbLate = sizeof(T) == 8 && S == 2;
   constexpr bool kP4Early = sizeof(T) == 4;  // phase 4 before the barrier after phase 3
   UIn Un1[5], Ur1[5];
@@ -582,7 +589,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     __syncthreads();
     {
       const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-      if (!kAhead && t + 1 <= tlast && warm_on)  // plane t+1 into L2 (FP64)
+      if (sizeof(T) == 8 && t + 1 <= tlast && warm_on)  // plane t+1 into L2 (FP64)
         __builtin_amdgcn_global_load_lds(
             (gptr<const unsigned int>)(Uin + (long long)(t + 1) * PL + warm_e), &s_sink[wave * 64],
             4, 0, 0);
@@ -590,7 +597,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
       // registers here, FP64 warms its lines in L2
       if (stage == 2 && t - 2 >= kb && t - 2 < ke) {
         const long long nb = (long long)(t - 2) * PL;
-        if constexpr (kAhead) {
+        if constexpr (kUbEarly) {
           if (own_valid) {
 #pragma unroll
             for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
@@ -886,7 +893,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
           if (stage == 1) {
   #pragma unroll
             for (int c = 0; c < 5; c++) Un[c] = (double)Uw[2][c] + 0.5 * ph.dt * R[c];
-          } else if constexpr (kAhead || kUbLate) {
+          } else if constexpr (kUbEarly || kUbLate) {
   #pragma unroll
             for (int c = 0; c < 5; c++) Un[c] = Ub[c] + ph.dt * R[c];
           } else {
@@ -917,7 +924,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     };
     if constexpr (kP4Early) phase4();
     __syncthreads();
-    if constexpr (kAhead) {
+    if constexpr (kAhead && sizeof(T) == 4) {
       if (t + 1 <= tlast) load_plane(t + 1);
     }
     if constexpr (kUbLate) {
@@ -947,6 +954,9 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
         Sg[1][c + 1] += (d[c][1] - d[c][0]) * hx + (d[c][3] - d[c][2]) * hy;
     }
 
+    if constexpr (kAhead && sizeof(T) == 8) {
+      if (t + 1 <= tlast) load_plane(t + 1);
+    }
     if constexpr (!kP4Early) phase4();
     // No barrier here.  The next plane's phase 1 writes sl, spo, spr, srz
     // and sub, which this plane last read in phases 2 and 3, before the
