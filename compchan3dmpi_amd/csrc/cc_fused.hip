@@ -33,12 +33,13 @@
 // vector-load wait drains the state loads in flight.  FP32 prefetches the
 // next plane into registers after phase 3 and keeps its z window in FP32,
 // which holds it to 168 VGPRs and three waves per SIMD; in stage 2 it loads
-// the base state U^n of phase 4 after the phase-1 barrier.  FP64 stage 1
-// loads the next plane into registers just before phase 4, which needs no
-// base state from memory.  FP64 stage 2, out of registers, warms the next
-// plane in L2 with unread LDS-DMA loads, one per cache line, during phases
-// 2-3, warms U^n the same way and loads it after the phase-3 barrier, so
-// that phase 3b hides the L2 hit.  The batched kernel warms in both stages.
+// the base state U^n of phase 4 after the phase-1 barrier.  FP64 warms the
+// next plane in L2 with unread LDS-DMA loads, one per cache line, during
+// phases 2-3.  Stage 1 (one-instance kernel) then loads it into registers
+// just before phase 4, which needs no base state from memory; stage 2, out
+// of registers, reads it from L2 in phase 1, and warms U^n the same way
+// and loads it after the phase-3 barrier, so that phase 3b hides the L2
+// hit.  The batched kernel reads the next plane from L2 in both stages.
 #include <type_traits>
 
 #include "cc_device.h"
