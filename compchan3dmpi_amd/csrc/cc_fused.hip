@@ -34,8 +34,8 @@
 // next plane into registers after phase 3 and keeps its z window in FP32,
 // which holds it to 168 VGPRs and three waves per SIMD; in stage 2 it loads
 // the base state U^n of phase 4 after the phase-1 barrier.  FP64 warms the
-// next plane in L2 with unread LDS-DMA loads, one per cache line, during
-// phases 2-3.  Stage 1 (one-instance kernel) then loads it into registers
+// next plane in L2 with 4-byte loads whose values are discarded, one per
+// cache line, during phases 2-3.  Stage 1 (one-instance kernel) then loads it into registers
 // just before phase 4, which needs no base state from memory; stage 2, out
 // of registers, reads it from L2 in phase 1, and warms U^n the same way
 // and loads it after the phase-3 barrier, so that phase 3b hides the L2
@@ -252,12 +252,6 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   __shared__ T sub[5 * NR];     // U(5) of plane t (phase 2)
   __shared__ T sfg[4 * (NFX + NGY)];  // fD and gD of plane t-1 (phases 3, 3b)
   __shared__ T srz[3][3][NZR];  // u, v, w on the ring-1 columns, three planes
-  // L2 warming (FP64, which has no registers to prefetch into): LDS-DMA
-  // loads that nothing reads pull the cache lines of the next plane, and in
-  // stage 2 of phase 4's base state, into L2 while phases 2 and 3 compute;
-  // the barrier after phase 3 waits for them.  One 4-byte word per lane
-  // lands here.
-  __shared__ unsigned int s_sink[NT];
 
   const auto& g = a.g;
   const auto& ph = a.ph;
@@ -331,13 +325,19 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
   // after the y-ring ones, so that the two extra passes run on different
   // SIMDs (waves are dealt to the SIMDs round-robin).  Computed here rather
   // than per plane, the index keeps the LDS accesses attributable to their
-  // arrays, which lets the compiler keep LDS-DMA loads in flight past them.
-  // FP64: the next plane is warmed in L2 by one LDS-DMA load per cache line
-  // of its region (own and halo rows, five fields), not one per node: the
-  // CU moves LDS-DMA data at ~10 bytes per clock, and one load per node and
-  // field took ~8 % of the plane time.  The probes lie at most 16 nodes (one
-  // 128-byte line) apart along each of the TY + 4 rows, both ends included;
-  // threads past the probe count idle.
+  // arrays.
+  // L2 warming (FP64, which has no registers to prefetch into): the cache
+  // lines of the next plane, and in stage 2 of phase 4's base state, are
+  // pulled into L2 while phases 2 and 3 compute, by one 4-byte load per
+  // cache line of the region (own and halo rows, five fields), not one per
+  // node (one load per node and field took ~8 % of the plane time).  The
+  // loaded word is held in a register that nothing reads until after the
+  // phase-3 barrier.  Loads into LDS (LDS-DMA) would need no register, but
+  // the compiler drains every LDS-DMA load in flight before the next LDS
+  // write (phase 3's fD/gD) and before the barrier: the FP64 stage ran 2.4 %
+  // slower with them.  The probes lie at most 16 nodes (one 128-byte line)
+  // apart along each of the TY + 4 rows, both ends included; threads past
+  // the probe count idle.
   constexpr int kProbeX = (TX + 4 + 15) / 16 + 1, kProbeRow = (TY + 4) * kProbeX;
   static_assert(sizeof(T) == 4 || 5 * kProbeRow <= NT, "one probe per thread");
   bool warm_on = false;
@@ -601,12 +601,10 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     asm volatile("" ::"s"(jac_t), "s"(iJ_t), "s"(iJ_p), "s"(iJ_q), "s"(rb_q));
     __syncthreads();
     __builtin_amdgcn_s_setprio(2);
+    unsigned warm_w = 0, warm_b = 0;  // the warming loads' words (discarded)
     {
-      const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
       if (sizeof(T) == 8 && t + 1 <= tlast && warm_on)  // plane t+1 into L2 (FP64)
-        __builtin_amdgcn_global_load_lds(
-            (gptr<const unsigned int>)(Uin + (long long)(t + 1) * PL + warm_e), &s_sink[wave * 64],
-            4, 0, 0);
+        warm_w = *(gptr<const unsigned int>)(Uin + (long long)(t + 1) * PL + warm_e);
       // the base state of phase 4 (stage 2): the mixed kernel loads it into
       // registers here, FP64 warms its lines in L2
       if (stage == 2 && t - 2 >= kb && t - 2 < ke) {
@@ -617,13 +615,14 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
             for (int c = 0; c < 5; c++) Ub[c] = *at_b<kSaddr>(Ubase + c * F + nb, o_own, bd_own);
           }
         } else if constexpr (kUbLate) {
-          if (bwarm_on)
-            __builtin_amdgcn_global_load_lds((gptr<const unsigned int>)(Ubase + nb + bwarm_e),
-                                             &s_sink[wave * 64], 4, 0, 0);
+          if (bwarm_on) warm_b = *(gptr<const unsigned int>)(Ubase + nb + bwarm_e);
         }
       }
     }
 
+    // (the warming loads stay here: nothing would keep the scheduler from
+    // sinking them towards their only use)
+    __builtin_amdgcn_sched_barrier(0);
     // ---- phase 2: x, y convective fluxes of plane t (main_v1.9.f90:333-407)
     // Each direction first gathers its whole stencil from LDS, the node's own
     // state coming from the z window and phase 1's registers, and only then
@@ -942,6 +941,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
     if constexpr (kP4Early) phase4();
     __syncthreads();
     __builtin_amdgcn_s_setprio(1);
+    if constexpr (sizeof(T) == 8) asm volatile("" ::"v"(warm_w), "v"(warm_b));
     if constexpr (kAhead && sizeof(T) == 4) {
       if (t + 1 <= tlast) load_plane(t + 1);
     }
@@ -990,7 +990,7 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
 // of tiles and their shared halos stay in that XCD's L2.
 // The FP32 kernels are held to 168 VGPRs, three waves per SIMD: three blocks
 // of a 32x8 tile share a CU (128 KB of LDS).  FP64 runs at 256 VGPRs, one
-// 512-thread block per CU, which its 143 KB of LDS allows anyway.
+// 512-thread block per CU, which its 137 KB of LDS allows anyway.
 template <typename T, int TX, int TY, int S>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
 k_stage_fused(Inst a, int stage, int step) {
@@ -1258,7 +1258,7 @@ __global__ void __launch_bounds__(512) k_wall_batched(const Inst* __restrict__ i
   wall_tile(insts[blockIdx.z >> 1], stage, step, blockIdx.z & 1);
 }
 
-// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (143 KB of
+// Tile shapes compiled for the stage kernel.  FP64 keeps 32x16 (137 KB of
 // LDS, one block per CU); FP32 also has the smaller shapes the batch
 // autotuner and choose_tiling choose between.
 void tile_of(int variant, int& tx, int& ty) {
