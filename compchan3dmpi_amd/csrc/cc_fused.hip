@@ -892,17 +892,44 @@ __device__ __forceinline__ void stage_tile(I& a, const int stage_arg, const int 
           H[c] = apply_w<T>(wz, Uw[0][c], Uw[1][c], Uw[2][c], Uw[3][c], Uw[4][c], T(0));
         // (the z divergence of hD is already in Sg[0])
         double R[5];
-        const double rb = rb_q;
-        const double src1 = 0.5 * tdiff * rb;
-        // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
-        const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
-  #pragma unroll
-        for (int c = 0; c < 5; c++) R[c] = -(double)(Sg[0][c] + H[c]);
-        R[1] += src1;
-        R[4] += src4;
-        // buoyancy -Ri rho along z and its work (adds -0 when Ri = 0: no branch)
-        R[3] += -ph.Ri * (double)Uw[2][0];
-        R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
+        if constexpr (sizeof(T) == 4) {
+          // The FP32 kernels' flux sums are FP32, so the forcing and buoyancy
+          // terms join the residual in FP32 too.  Stage 1 also updates in
+          // FP32, as U* is stored in FP32; stage 2 adds dt R to the FP64
+          // base state in FP64 (below).
+          const T s1 = T(0.5 * tdiff * rb_q);
+          T Rt[5];
+#pragma unroll
+          for (int c = 0; c < 5; c++) Rt[c] = -(Sg[0][c] + H[c]);
+          Rt[1] += s1;
+          Rt[4] += T(ph.ggM) * zU[0] * s1;
+          Rt[3] += T(-ph.Ri) * Uw[2][0];
+          Rt[4] += T(-ph.ggM * ph.Ri) * Uw[2][3];
+          if constexpr (kOut32) {
+            if (own_valid) {
+              const long long n = (long long)q * PL;
+              const T hdt = T(0.5 * ph.dt);
+#pragma unroll
+              for (int c = 0; c < 5; c++)
+                *at_b<kSaddr>((float*)Uout + c * F + n, o_own, bd_own / 2) = Uw[2][c] + hdt * Rt[c];
+            }
+            return;
+          }
+#pragma unroll
+          for (int c = 0; c < 5; c++) R[c] = (double)Rt[c];
+        } else {
+          const double rb = rb_q;
+          const double src1 = 0.5 * tdiff * rb;
+          // u of plane q as phase 1 recovered it (zU holds plane t-2 = q)
+          const double src4 = ph.ggM * (double)zU[0] * 0.5 * tdiff * rb;
+#pragma unroll
+          for (int c = 0; c < 5; c++) R[c] = -(double)(Sg[0][c] + H[c]);
+          R[1] += src1;
+          R[4] += src4;
+          // buoyancy -Ri rho along z and its work (adds -0 when Ri = 0: no branch)
+          R[3] += -ph.Ri * (double)Uw[2][0];
+          R[4] += -ph.ggM * ph.Ri * (double)Uw[2][3];
+        }
         if (own_valid) {
           const long long n = (long long)q * PL;
           double Un[5];
